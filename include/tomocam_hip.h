@@ -1,0 +1,215 @@
+/*
+ * C-ABI of the MI355X-native tomoCAM MBIR hot path (libtomocam_hip.so).
+ *
+ * Conventions
+ *   - Every function returns an int status (TC_OK = 0); on failure
+ *     tc_last_error() returns a thread-local message.  Nothing calls exit().
+ *   - Pointers named d_* are device pointers on the current HIP device and
+ *     h_* are host pointers.  Arrays are C-contiguous float32:
+ *       image stacks  [nslices][npix][npix]
+ *       sinograms     [nslices][nproj][nrays]
+ *   - `stream` is a hipStream_t (NULL = the default stream).  Functions that
+ *     return a host scalar, and the MBIR drivers, synchronise `stream`;
+ *     others are asynchronous on `stream`.
+ *   - The library never frees caller memory.  Opaque handles (tc_psf) are
+ *     created and destroyed explicitly.
+ *
+ * Each declaration names the reference interface it replaces
+ * (file:line in lbl-camera/tomocam).
+ */
+#ifndef TOMOCAM_HIP_H
+#define TOMOCAM_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TC_OK 0
+#define TC_ERR_INVALID 1  /* bad argument or shape */
+#define TC_ERR_HIP 2      /* HIP runtime error */
+#define TC_ERR_NOMEM 3    /* device allocation failed */
+#define TC_ERR_INTERNAL 4 /* anything else */
+#define TC_ERR_VALUE 5    /* numerically invalid input (all-zero sinogram) */
+
+typedef struct tc_psf_s* tc_psf;
+
+/* Message of the last failure on this thread ("" if none). */
+const char* tc_last_error(void);
+
+/* Library version. */
+int tc_version(int* major, int* minor, int* patch);
+
+/* cTomocam.set_num_of_gpus (src/pyTomocam.cpp:141-142, src/machine.h:48-54).
+ * The library runs on one GPU: n must be >= 1 and is otherwise ignored. */
+int tc_set_num_of_gpus(int n);
+
+/* ---------------------------------------------------------------------- */
+/* projectors                                                               */
+/* ---------------------------------------------------------------------- */
+
+/* tomocam::project (src/radon.cpp:70-96, single device src/forward.cpp:33-52):
+ * d_sino[S][P][N] = Re(fftshift(IFFT(ifftshift(A x))))/N^2 with A the type-2
+ * NUFFT on the polar grid r_j = 2 pi (j - N/2)/(N - 1) of h_angles[P]. */
+int tc_project(const float* d_img, float* d_sino, int nslices, int npix,
+               const float* h_angles, int nproj, void* stream);
+
+/* tomocam::backproject (src/iradon.cpp:75-102, single device
+ * src/backward.cpp:42-67): exact adjoint of tc_project; with fbp != 0 the
+ * Ram-Lak ramp |k - N/2|/N is applied first (src/gpu/ramlak.cu:14-31).
+ * d_sino[S][P][N] -> d_img[S][N][N]. */
+int tc_backproject(const float* d_sino, float* d_img, int nslices, int nproj,
+                   int npix, const float* h_angles, int fbp, void* stream);
+
+/* tomocam::gradient (src/gradient.cpp:35-103): d_g = Re(A^H A x)/N^3 - d_sinoT
+ * by a type-2 and a type-1 NUFFT (the PSF-free normal operator). */
+int tc_gradient(const float* d_x, const float* d_sinoT, float* d_g,
+                int nslices, int npix, const float* h_angles, int nproj,
+                void* stream);
+
+/* tomocam::function_value (src/error.cpp:36-88): *h_out = sum (P x - y)^2
+ * (FP64 accumulation). */
+int tc_function_value(const float* d_x, const float* d_sino, int nslices,
+                      int npix, const float* h_angles, int nproj,
+                      double* h_out, void* stream);
+
+/* tomocam::xerror (src/xerror.cpp:37-85): *h_out = sum (a - b)^2. */
+int tc_xerror(const float* d_a, const float* d_b, long n, double* h_out,
+              void* stream);
+
+/* tomocam::preproc (src/preproc.cpp:73-108): zero-pad each sinogram row by
+ * npad on both sides and roll by the centre offset.  *h_npad receives npad
+ * (output width N + 2 npad); call with d_out == NULL to query it. */
+int tc_preproc(const float* d_sino, float* d_out, int nslices, int nproj,
+               int nrays, float center, int* h_npad, void* stream);
+
+/* tomocam::postproc (src/postproc.cpp:69-102): crop [S][n][n] to the centred
+ * [S][npixels][npixels]. */
+int tc_postproc(const float* d_img, float* d_out, int nslices, int n,
+                int npixels, void* stream);
+
+/* tomocam::pad2d (src/padding.cpp:37-91): zero-pad [S][n][n] by `padding`
+ * in total, symmetrically, to [S][n+padding][n+padding]. */
+int tc_pad2d(const float* d_img, float* d_out, int nslices, int n,
+             int padding, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Toeplitz (point-spread-function) normal operator                         */
+/* ---------------------------------------------------------------------- */
+
+/* PointSpreadFunction<T>(grid) (src/toeplitz.h:51-73): builds the PSF of the
+ * polar geometry (h_angles[nproj], npix rays per angle) with a type-1 NUFFT
+ * on (2 npix - 1)^2 modes, and its real spectrum. */
+int tc_psf_create(tc_psf* out, const float* h_angles, int nproj, int npix,
+                  void* stream);
+
+/* Same object from an explicit real PSF array d_psf[2n-1][2n-1] (PSF(m) at
+ * index m + n - 1), e.g. one computed elsewhere. */
+int tc_psf_create_from_array(tc_psf* out, const float* d_psf, int n,
+                             void* stream);
+
+/* Copies the real PSF array [2n-1][2n-1] of `p` into d_psf (device). */
+int tc_psf_get_array(tc_psf p, float* d_psf, void* stream);
+
+/* ~PointSpreadFunction (src/toeplitz.h:75-81). */
+int tc_psf_destroy(tc_psf p);
+
+/* Image width n and FFT size M used by the convolution. */
+int tc_psf_info(tc_psf p, int* n, int* fft_size);
+
+/* PointSpreadFunction::convolve (src/toeplitz.h:95-101):
+ * d_out = T d_x = sum_q x[q] PSF(a - q) / n^3 per slice.  If h_xTx is not
+ * NULL it receives x . T x (FP64, computed in the spectral domain) and the
+ * call synchronises `stream`. */
+int tc_toeplitz_apply(tc_psf p, const float* d_x, float* d_out, int nslices,
+                      double* h_xTx, void* stream);
+
+/* tomocam::gradient2 (src/gradient2.cpp:36-100, src/tomocam.h:85-86):
+ * d_g = T d_x - d_sinoT. */
+int tc_gradient2(tc_psf p, const float* d_x, const float* d_sinoT, float* d_g,
+                 int nslices, void* stream);
+
+/* tomocam::function_value2 (src/error2.cpp:40-94, src/tomocam.h:109-110):
+ * *h_out = sino_sq + sum (x . T x - 2 x . sinoT), FP64 accumulation. */
+int tc_function_value2(tc_psf p, const float* d_x, const float* d_sinoT,
+                       int nslices, double sino_sq, double* h_out,
+                       void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* qGGMRF regulariser                                                       */
+/* ---------------------------------------------------------------------- */
+
+/* tomocam::add_total_var2 (src/tv_update.cpp:75-106) with qGGMRF_kernel
+ * (src/gpu/cuTV2.cu:32-93): d_g += sum_d w_d rho'(x_v - x_clamp(v+d)),
+ * q = 2, c = 0.001, replicate-at-edge clamp on all three axes. */
+int tc_add_total_var2(const float* d_x, float* d_g, int nslices, int nrows,
+                      int ncols, float sigma, float p, void* stream);
+
+/* tomocam::add_total_var (src/tvd_update.cpp:82-117, tvd_update_kernel
+ * src/gpu/cuTVD.cu:32-190): the same arithmetic as add_total_var2. */
+int tc_add_total_var(const float* d_x, float* d_g, int nslices, int nrows,
+                     int ncols, float sigma, float p, void* stream);
+
+/* gpu::add_tv_hessian (src/gpu/tv_hessian.cu:53-65).  The reference adds
+ * sum_d w_d q / (sigma^q c) to a device copy that is discarded, so the host
+ * array is unchanged; this mirrors that (h_g is not modified) and returns
+ * the value it would have added in *h_value (may be NULL). */
+int tc_add_tv_hessian(float* h_g, long n, float sigma, float* h_value);
+
+/* ---------------------------------------------------------------------- */
+/* MBIR iteration                                                           */
+/* ---------------------------------------------------------------------- */
+
+/* Optimizer::run2 (src/optimize.h:96-154) with the mbir2 objective
+ * (src/mbir2.cpp:122-136): accelerated gradient with backtracking on
+ * 0.5 ||P x - y||^2 + qGGMRF, device resident.  d_x0, d_sinoT, d_out are
+ * [nslices][n][n] with n = PSF width; sino_sq = ||y||^2.  tol/xtol are
+ * accepted and ignored like the reference.  refresh_every = k recomputes
+ * T(sol) and T(x) every k accepted steps (0 = never).  Optional outputs:
+ * h_err[num_iters], h_xerr[num_iters] (the reference's printed error and
+ * x-err per iteration); h_stats[4] = {accepted, attempts, backtracks,
+ * Toeplitz applications}.  Synchronises `stream`. */
+int tc_run2(tc_psf p, const float* d_x0, const float* d_sinoT, float* d_out,
+            int nslices, int num_iters, double step, float sigma, float p_exp,
+            double sino_sq, int refresh_every, int verbose, double* h_err,
+            double* h_xerr, int* h_stats, void* stream);
+
+/* tomocam::mbir2 (src/mbir2.cpp:40-145) without an initial guess, as called
+ * by cTomocam.mbir (src/pyTomocam.cpp:116-134): normalise by max(sino), FBP
+ * initial guess, pad/centre, P^T y, PSF, step = min(1, 1/L), run2, crop.
+ * d_sino[S][P][N] -> d_recon[S][N][N], both on the device.  tol/xtol are
+ * ignored like the reference; an all-zero sinogram returns TC_ERR_VALUE.
+ * Optional h_info[6] = {lipschitz, step, ||y||^2, max(sino), padded width,
+ * accepted iterations}.  Synchronises `stream`. */
+int tc_mbir2(const float* d_sino, float* d_recon, int nslices, int nproj,
+             int nrays, const float* h_angles, float center, int num_iters,
+             float sigma, float tol, float xtol, int refresh_every,
+             int verbose, double* h_info, void* stream);
+
+/* Same with host arrays (copies in and out, like the reference's DArray
+ * interface). */
+int tc_mbir2_host(const float* h_sino, float* h_recon, int nslices, int nproj,
+                  int nrays, const float* h_angles, float center,
+                  int num_iters, float sigma, float tol, float xtol,
+                  int verbose);
+
+/* ---------------------------------------------------------------------- */
+/* instrumentation                                                          */
+/* ---------------------------------------------------------------------- */
+
+/* Kernel classes: 0 Toeplitz rows (forward), 1 Toeplitz columns,
+ * 2 Toeplitz rows (inverse), 3 MBIR gradient (stencil + assembly),
+ * 4 MBIR commit, 5 standalone qGGMRF.  When enabled, launches of that class
+ * are bracketed by HIP events on their stream. */
+int tc_ktimer_enable(int kernel_class, int on);
+
+/* Sum of the event-timed durations (ms) and number of launches of a class
+ * since the previous collect. */
+int tc_ktimer_collect(int kernel_class, double* total_ms, long* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TOMOCAM_HIP_H */

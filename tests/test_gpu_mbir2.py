@@ -1,0 +1,88 @@
+"""Full mbir2 pipeline (normalise, FBP guess, pad, P^T y, PSF, step, run2,
+crop) on the GPU vs the FP64 oracle, plus its error behaviour.
+
+Tolerance: relative L2 <= 1e-4 on the reconstruction;
+set-up scalars (Lipschitz estimate, ||y||^2) to 1e-5 relative.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import tomocam_oracle as O
+from tomocam_amd import phantom as PH
+from conftest import rel_l2, announce
+
+pytestmark = pytest.mark.gpu
+
+
+def _sino(N, P, S, seed, kind="ellipses", noise=0.01):
+    ang = PH.angles(P)
+    if kind == "shepp":
+        img = PH.phantom_stack(N, S, "shepp").astype(np.float64)
+    else:
+        img = np.stack([PH.random_ellipses(N, seed + s) for s in range(S)])
+    sino = O.project(img, ang.astype(np.float64), exact=N <= 64)
+    sino = sino + np.random.default_rng(seed).normal(
+        0, noise * sino.max(), sino.shape)
+    return ang, sino
+
+
+@pytest.mark.parametrize("center_shift", [0.0, 2.6])
+def test_mbir2_small_vs_oracle(torch_cuda, center_shift):
+    torch = torch_cuda
+    from tomocam_amd import gpu
+    N, P, S, iters, sigma = 32, 24, 3, 20, 10.0
+    ang, sino = _sino(N, P, S, 5)
+    center = N / 2 + center_shift
+    st = O.mbir2_setup(sino, ang.astype(np.float64), center, exact=True)
+    ref = O.mbir2(sino, ang.astype(np.float64), center, iters, sigma,
+                  setup=st)
+    announce(f"mbir2 small shift={center_shift}")
+    out, info = gpu.mbir(torch.tensor(sino, dtype=torch.float32).cuda(), ang,
+                         center, iters, sigma)
+    assert info["iterations"] == iters
+    assert info["padded"] == st.sino2.shape[-1]
+    assert abs(info["lipschitz"] - st.lipschitz) <= 1e-5 * st.lipschitz
+    assert abs(info["sino_norm"] - st.sino_norm) <= 1e-5 * st.sino_norm
+    assert info["step"] == min(1.0, 1.0 / info["lipschitz"])
+    assert rel_l2(out.cpu().numpy(), ref) < 1e-4
+
+
+def test_mbir2_config0_shape(torch_cuda):
+    """The CPU-path workload: 4 x 256^2 Shepp-Logan, 180 angles, centre 128,
+    1% noise, 50 iterations.  Compared at sigma = 10 (converging) against
+    the FP64 oracle with the single-Toeplitz run2 restatement."""
+    torch = torch_cuda
+    from tomocam_amd import gpu
+    N, P, S, iters, sigma = 256, 180, 4, 50, 10.0
+    ang, sino = _sino(N, P, S, 7, kind="shepp")
+    a64 = ang.astype(np.float64)
+    st = O.mbir2_setup(sino, a64, N / 2)
+    T = lambda v: O.toeplitz_circulant(v, st.psf, O.smooth_size(2 * st.x0.shape[-1] - 1))
+    reg = lambda v: O.qggmrf_grad(v, sigma, 1.2)
+    rec = O.run2_linear(st.x0, T, st.sinoT, reg, st.sino_norm, iters, st.step)
+    ref = O.postproc(rec, N)
+    announce("mbir2 config0 shape")
+    out, info = gpu.mbir(torch.tensor(sino, dtype=torch.float32).cuda(), ang,
+                         N / 2, iters, sigma)
+    assert info["padded"] == 362
+    assert rel_l2(out.cpu().numpy(), ref) < 1e-4
+
+
+def test_mbir2_zero_sinogram_raises(torch_cuda):
+    torch = torch_cuda
+    from tomocam_amd import gpu
+    sino = torch.zeros((1, 8, 16), dtype=torch.float32).cuda()
+    announce("mbir2 zero sinogram")
+    with pytest.raises(ValueError, match="ZERO max"):
+        gpu.mbir(sino, PH.angles(8), 8.0, 2, 1.0)
+
+
+def test_mbir2_bad_angles_raises(torch_cuda):
+    torch = torch_cuda
+    from tomocam_amd import gpu
+    sino = torch.ones((1, 8, 16), dtype=torch.float32).cuda()
+    with pytest.raises(ValueError):
+        gpu.mbir(sino, PH.angles(7), 8.0, 2, 1.0)

@@ -1,0 +1,134 @@
+"""NUFFT projectors, PSF and pre/post-processing on the GPU vs the oracle.
+
+Tolerances: relative L2 <= 1e-5 for the projectors and PSF against the exact
+non-uniform DFT (FP32 arithmetic, ES kernel w = 8 with ~1.4e-7 kernel
+error); adjoint identity to 1e-6 relative at full workload sizes; index
+operations (pre/post-processing) bit-exact.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import tomocam_oracle as O
+from tomocam_amd import phantom as PH
+from conftest import rel_l2, announce
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(torch, a):
+    return torch.tensor(np.ascontiguousarray(a), dtype=torch.float32).cuda()
+
+
+@pytest.mark.parametrize("N,P,S", [(16, 9, 2), (33, 20, 1), (64, 50, 3),
+                                    (15, 7, 1)])
+def test_project_backproject_exact(torch_cuda, N, P, S):
+    torch = torch_cuda
+    from tomocam_amd import gpu
+    ang = PH.angles(P)
+    rng = np.random.default_rng(N * 100 + P)
+    img = rng.standard_normal((S, N, N))
+    sino = rng.standard_normal((S, P, N))
+    a64 = ang.astype(np.float64)
+    announce(f"project/backproject N={N} P={P} S={S}")
+    got = gpu.project(_t(torch, img), ang).cpu().numpy()
+    assert rel_l2(got, O.project(img, a64, exact=True)) < 1e-5
+    got = gpu.backproject(_t(torch, sino), ang).cpu().numpy()
+    assert rel_l2(got, O.backproject(sino, a64, exact=True)) < 1e-5
+    got = gpu.backproject(_t(torch, sino), ang, fbp=True).cpu().numpy()
+    assert rel_l2(got, O.backproject(sino, a64, fbp=True, exact=True)) < 1e-5
+
+
+def test_project_config0_geometry(torch_cuda):
+    """256^2 Shepp-Logan, 180 angles (the CPU-path workload's geometry)
+    against the FP64 ES-kernel oracle (w = 14)."""
+    torch = torch_cuda
+    from tomocam_amd import gpu
+    N, P = 256, 180
+    ang = PH.angles(P)
+    img = PH.phantom_stack(N, 1, "shepp").astype(np.float64)
+    ref = O.project(img, ang.astype(np.float64))
+    announce("project 256x180")
+    got = gpu.project(_t(torch, img), ang).cpu().numpy()
+    assert rel_l2(got, ref) < 1e-5
+    s = np.random.default_rng(1).standard_normal((1, P, N))
+    refb = O.backproject(s, ang.astype(np.float64), fbp=True)
+    gotb = gpu.backproject(_t(torch, s), ang, fbp=True).cpu().numpy()
+    assert rel_l2(gotb, refb) < 1e-5
+
+
+@pytest.mark.parametrize("N,P,S", [(256, 180, 2), (362, 180, 1),
+                                    (1024, 720, 1), (1454, 720, 1)])
+def test_adjoint_identity(torch_cuda, N, P, S):
+    """<P x, s> = <x, P^T s> (exact in exact arithmetic for every N)."""
+    torch = torch_cuda
+    from tomocam_amd import gpu
+    ang = PH.angles(P)
+    rng = np.random.default_rng(N)
+    x = _t(torch, rng.standard_normal((S, N, N)))
+    s = _t(torch, rng.standard_normal((S, P, N)))
+    announce(f"adjoint N={N} P={P}")
+    px = gpu.project(x, ang)
+    pts = gpu.backproject(s, ang)
+    lhs = float(torch.sum(px.double() * s.double()))
+    rhs = float(torch.sum(x.double() * pts.double()))
+    scale = float(torch.linalg.norm(px.double()) * torch.linalg.norm(s.double()))
+    assert abs(lhs - rhs) <= 1e-6 * scale
+
+
+@pytest.mark.parametrize("n,P", [(16, 9), (33, 20), (48, 30)])
+def test_psf_matches_oracle(torch_cuda, n, P):
+    torch = torch_cuda
+    from tomocam_amd import gpu
+    ang = PH.angles(P)
+    ref = O.psf(ang.astype(np.float64), n, exact=True)
+    announce(f"psf n={n} P={P}")
+    psf = gpu.PointSpreadFunction(ang, n)
+    got = psf.array().cpu().numpy()
+    assert rel_l2(got, ref) < 1e-5
+    # the Toeplitz operator built from it equals Re(A^H A x)/n^3
+    x = np.random.default_rng(n).standard_normal((2, n, n))
+    tx = psf.convolve(_t(torch, x)).cpu().numpy()
+    g = O.gradient_nufft(x, 0 * x, ang.astype(np.float64), exact=True)
+    assert rel_l2(tx, g) < 1e-5
+
+
+def test_gradient_and_function_value(torch_cuda):
+    torch = torch_cuda
+    from tomocam_amd import gpu
+    N, P, S = 32, 24, 2
+    ang = PH.angles(P)
+    a64 = ang.astype(np.float64)
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((S, N, N))
+    b = rng.standard_normal((S, N, N))
+    y = rng.standard_normal((S, P, N))
+    announce("gradient/function_value N=32")
+    g = gpu.gradient(_t(torch, x), _t(torch, b), ang).cpu().numpy()
+    assert rel_l2(g, O.gradient_nufft(x, b, a64, exact=True)) < 1e-5
+    fv = gpu.function_value(_t(torch, x), _t(torch, y), ang)
+    ref = O.function_value(x, y, a64, exact=True)
+    assert abs(fv - ref) <= 1e-5 * ref
+    xe = gpu.xerror(_t(torch, x), _t(torch, b))
+    assert abs(xe - np.sum((x.astype(np.float32) - b.astype(np.float32)) ** 2)) <= 1e-6 * xe
+
+
+@pytest.mark.parametrize("N,center", [(32, 16.0), (33, 16.5), (40, 27.0),
+                                      (64, 20.4), (256, 128.0), (31, 3.0)])
+def test_pre_post_processing_exact(torch_cuda, N, center):
+    torch = torch_cuda
+    from tomocam_amd import gpu
+    rng = np.random.default_rng(N)
+    sino = rng.standard_normal((2, 5, N)).astype(np.float32)
+    announce(f"preproc N={N} center={center}")
+    got = gpu.preproc(_t(torch, sino), center).cpu().numpy()
+    ref = O.preproc(sino, center)
+    assert got.shape == ref.shape
+    assert np.array_equal(got, ref)
+    n2 = ref.shape[-1]
+    img = rng.standard_normal((2, n2, n2)).astype(np.float32)
+    assert np.array_equal(gpu.postproc(_t(torch, img), N).cpu().numpy(),
+                          O.postproc(img, N))
+    small = rng.standard_normal((2, N, N)).astype(np.float32)
+    assert np.array_equal(gpu.pad2d(_t(torch, small), n2 - N).cpu().numpy(),
+                          O.pad2d(small, n2 - N))

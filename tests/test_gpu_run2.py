@@ -1,0 +1,109 @@
+"""Device-resident Optimizer::run2 vs the FP64 oracle restatement.
+
+The set-up (x0, sinoT, PSF, ||y||^2, step) comes from the oracle so this
+pins the iteration alone.  Tolerance: relative L2 <= 1e-4 on the solution
+and 1e-5 on the per-iteration error history.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import tomocam_oracle as O
+from tomocam_amd import phantom as PH
+from conftest import rel_l2, announce
+
+pytestmark = pytest.mark.gpu
+
+
+def make_setup(N=32, P=24, S=3, seed=11):
+    ang = PH.angles(P).astype(np.float64)
+    img = np.stack([PH.random_ellipses(N, seed + s) for s in range(S)])
+    sino = O.project(img, ang, exact=True)
+    sino = sino + np.random.default_rng(seed).normal(0, 0.01 * sino.max(),
+                                                     sino.shape)
+    return O.mbir2_setup(sino, ang, N / 2, exact=True)
+
+
+@pytest.fixture(scope="module")
+def small_setup():
+    return make_setup()
+
+
+def oracle_run2(st, x0, iters, step, sigma):
+    log = O.Run2Log()
+
+    def grad_fn(x):
+        return O.gradient2(x, st.sinoT, st.psf) + O.qggmrf_grad(x, sigma, 1.2)
+
+    def err_fn(x):
+        return math.sqrt(O.function_value2(x, st.sinoT, st.psf, st.sino_norm))
+
+    return O.run2(x0, grad_fn, err_fn, iters, step, log), log
+
+
+def gpu_run2(torch, st, x0, iters, step, sigma, refresh=10):
+    from tomocam_amd import gpu
+    psf = gpu.PointSpreadFunction(
+        psf_array=torch.tensor(st.psf, dtype=torch.float32).cuda())
+    x0t = torch.tensor(x0, dtype=torch.float32).cuda()
+    b = torch.tensor(st.sinoT, dtype=torch.float32).cuda()
+    return gpu.run2(psf, x0t, b, iters, step, sigma, 1.2, st.sino_norm,
+                    refresh_every=refresh)
+
+
+@pytest.mark.parametrize("sigma", [10.0, 100.0])
+def test_run2_matches_oracle(torch_cuda, small_setup, sigma):
+    st = small_setup
+    iters = 30
+    ref, log = oracle_run2(st, st.x0, iters, st.step, sigma)
+    announce(f"run2 sigma={sigma}")
+    out, err, xerr, stats = gpu_run2(torch_cuda, st, st.x0, iters, st.step,
+                                     sigma)
+    assert stats["iterations"] == iters
+    assert stats["backtracks"] == log.backtracks
+    assert rel_l2(out.cpu().numpy(), ref) < 1e-4
+    np.testing.assert_allclose(err, log.errors, rtol=1e-5)
+    np.testing.assert_allclose(xerr, log.xerrs, rtol=1e-3, atol=1e-6)
+
+
+def test_run2_backtracking(torch_cuda, small_setup):
+    """A smooth perturbation of a converged point with step 6 makes the
+    reference reject 4 candidates in two iterations; a rejected candidate
+    still becomes `sol` (src/optimize.h:120,127-128)."""
+    st = small_setup
+    sigma = 1e4
+    T = lambda v: O.toeplitz(v, st.psf)
+    reg = lambda v: O.qggmrf_grad(v, sigma, 1.2)
+    xs = O.run2_linear(st.x0, T, st.sinoT, reg, st.sino_norm, 150, 3.0)
+    x0 = xs + 1e-3
+    ref, log = oracle_run2(st, x0, 2, 6.0, sigma)
+    assert log.backtracks == 4
+    announce("run2 backtracking")
+    out, err, _, stats = gpu_run2(torch_cuda, st, x0, 2, 6.0, sigma)
+    assert stats["backtracks"] == log.backtracks
+    assert stats["attempts"] == 2 + log.backtracks
+    assert rel_l2(out.cpu().numpy(), ref) < 1e-4
+    np.testing.assert_allclose(err, log.errors, rtol=1e-4)
+
+
+@pytest.mark.parametrize("refresh", [0, 1, 10])
+def test_run2_refresh(torch_cuda, small_setup, refresh):
+    """With or without recomputing T(sol) the result meets the bound."""
+    st = small_setup
+    ref, _ = oracle_run2(st, st.x0, 40, st.step, 10.0)
+    announce(f"run2 refresh={refresh}")
+    out = gpu_run2(torch_cuda, st, st.x0, 40, st.step, 10.0, refresh)[0]
+    assert rel_l2(out.cpu().numpy(), ref) < 1e-4
+
+
+def test_run2_divergent_sigma_first_iterations(torch_cuda, small_setup):
+    """sigma = 0.1 (the benchmark's regulariser) makes the reference
+    algorithm diverge; the early trajectory still matches."""
+    st = small_setup
+    ref, log = oracle_run2(st, st.x0, 5, st.step, 0.1)
+    announce("run2 sigma=0.1")
+    out, err, _, _ = gpu_run2(torch_cuda, st, st.x0, 5, st.step, 0.1)
+    assert rel_l2(out.cpu().numpy(), ref) < 1e-4
+    np.testing.assert_allclose(err, log.errors, rtol=1e-4)

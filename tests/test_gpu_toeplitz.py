@@ -1,0 +1,89 @@
+"""Toeplitz operator on the GPU vs the FP64 oracle (through the C-ABI).
+
+Tolerance: relative L2 <= 2e-5 for T x (FP32 FFTs of length M ~ 2n), and
+relative 1e-5 for the spectral x . T x.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import tomocam_oracle as O
+from conftest import rel_l2, announce
+
+pytestmark = pytest.mark.gpu
+
+
+def _even_psf(n, seed):
+    """Synthetic real PSF on (2n-1)^2 that is even (PSF(m) = PSF(-m)), as
+    every polar-grid PSF is."""
+    r = np.random.default_rng(seed).standard_normal((2 * n - 1, 2 * n - 1))
+    return 0.5 * (r + r[::-1, ::-1])
+
+
+@pytest.mark.parametrize("n,P,S", [(15, 7, 2), (16, 9, 3), (33, 20, 1),
+                                    (48, 30, 2)])
+def test_toeplitz_polar_psf(torch_cuda, n, P, S):
+    torch = torch_cuda
+    from tomocam_amd import gpu
+    ang = np.arange(P) * np.pi / P
+    ps = O.psf(ang, n, exact=True)
+    x = np.random.default_rng(n).standard_normal((S, n, n))
+    ref = O.toeplitz(x, ps)
+    announce(f"toeplitz polar n={n} P={P} S={S}")
+    psf = gpu.PointSpreadFunction(psf_array=torch.tensor(ps, dtype=torch.float32).cuda())
+    xt = torch.tensor(x, dtype=torch.float32).cuda()
+    out, xtx = psf.convolve(xt, with_dot=True)
+    out = out.cpu().numpy()
+    assert rel_l2(out, ref) < 2e-5
+    # brute-force definition (tiny sizes)
+    if n <= 16:
+        assert rel_l2(out, O.toeplitz_bruteforce(x, ps)) < 2e-5
+    # spectral dot x . T x
+    assert abs(xtx - np.sum(x * ref)) <= 1e-5 * abs(np.sum(x * ref))
+
+
+@pytest.mark.parametrize("n,S", [(181, 2), (362, 2), (726, 1), (1454, 1)])
+def test_toeplitz_sizes(torch_cuda, n, S):
+    """FFT sizes of the workloads (M = 2n-1 rounded to a smooth size)."""
+    torch = torch_cuda
+    from tomocam_amd import gpu
+    ps = _even_psf(n, 7)
+    x = np.random.default_rng(n + 1).standard_normal((S, n, n))
+    ref = O.toeplitz(x, ps)
+    announce(f"toeplitz size n={n} S={S}")
+    psf = gpu.PointSpreadFunction(psf_array=torch.tensor(ps, dtype=torch.float32).cuda())
+    out, xtx = psf.convolve(torch.tensor(x, dtype=torch.float32).cuda(),
+                            with_dot=True)
+    assert rel_l2(out.cpu().numpy(), ref) < 2e-5
+    assert abs(xtx - np.sum(x * ref)) <= 1e-5 * np.sum(np.abs(x * ref))
+
+
+def test_gradient2_function_value2(torch_cuda):
+    torch = torch_cuda
+    from tomocam_amd import gpu
+    n, P, S = 32, 24, 2
+    ang = np.arange(P) * np.pi / P
+    ps = O.psf(ang, n, exact=True)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((S, n, n))
+    b = rng.standard_normal((S, n, n))
+    announce("gradient2/function_value2 n=32")
+    psf = gpu.PointSpreadFunction(psf_array=torch.tensor(ps, dtype=torch.float32).cuda())
+    xt = torch.tensor(x, dtype=torch.float32).cuda()
+    bt = torch.tensor(b, dtype=torch.float32).cuda()
+    g = psf.gradient2(xt, bt).cpu().numpy()
+    assert rel_l2(g, O.gradient2(x, b, ps)) < 2e-5
+    fv = psf.function_value2(xt, bt, 123.0)
+    ref = O.function_value2(x, b, ps, 123.0)
+    assert abs(fv - ref) <= 1e-5 * (abs(ref) + np.sum(np.abs(x * b)))
+
+
+def test_psf_roundtrip_and_info(torch_cuda):
+    torch = torch_cuda
+    from tomocam_amd import gpu
+    ps = _even_psf(20, 1)
+    psf = gpu.PointSpreadFunction(psf_array=torch.tensor(ps, dtype=torch.float32).cuda())
+    assert psf.n == 20
+    assert psf.fft_size >= 39 and psf.fft_size % 2 == 0
+    back = psf.array().cpu().numpy()
+    assert np.array_equal(back, ps.astype(np.float32))

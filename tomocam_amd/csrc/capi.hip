@@ -1,0 +1,358 @@
+// extern "C" entry points (declared in include/tomocam_hip.h).
+#include <cmath>
+#include <memory>
+#include <vector>
+
+#include "driver.h"
+#include "mbir.h"
+#include "nufft.h"
+#include "toeplitz.h"
+#include "vecops.h"
+
+namespace tc {
+void ktimer_enable(int kc, bool on);
+void ktimer_collect(int kc, double* total_ms, long* count);
+}  // namespace tc
+
+struct tc_psf_s {
+  std::unique_ptr<tc::Psf> psf;
+};
+
+using tc::guarded;
+
+static hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+static tc::Psf& get_psf(tc_psf p) {
+  TC_REQUIRE(p != nullptr && p->psf, "null PSF handle");
+  return *p->psf;
+}
+
+extern "C" {
+
+const char* tc_last_error(void) { return tc::last_error(); }
+
+int tc_version(int* major, int* minor, int* patch) {
+  return guarded([&] {
+    if (major) *major = 0;
+    if (minor) *minor = 1;
+    if (patch) *patch = 0;
+  });
+}
+
+int tc_set_num_of_gpus(int n) {
+  return guarded([&] {
+    TC_REQUIRE(n >= 1, "set_num_of_gpus: n must be >= 1");
+  });
+}
+
+int tc_project(const float* d_img, float* d_sino, int nslices, int npix,
+               const float* h_angles, int nproj, void* stream) {
+  return guarded([&] {
+    TC_REQUIRE(d_img && d_sino && h_angles, "null argument");
+    tc::project(d_img, d_sino, nslices, npix, h_angles, nproj,
+                as_stream(stream));
+  });
+}
+
+int tc_backproject(const float* d_sino, float* d_img, int nslices, int nproj,
+                   int npix, const float* h_angles, int fbp, void* stream) {
+  return guarded([&] {
+    TC_REQUIRE(d_img && d_sino && h_angles, "null argument");
+    tc::backproject(d_sino, d_img, nslices, nproj, npix, h_angles, fbp != 0,
+                    as_stream(stream));
+  });
+}
+
+int tc_gradient(const float* d_x, const float* d_sinoT, float* d_g,
+                int nslices, int npix, const float* h_angles, int nproj,
+                void* stream) {
+  return guarded([&] {
+    TC_REQUIRE(d_x && d_sinoT && d_g && h_angles, "null argument");
+    TC_REQUIRE(nslices >= 1 && npix >= 2 && nproj >= 1, "gradient: bad shape");
+    hipStream_t s = as_stream(stream);
+    tc::PolarNufft& nu = tc::nufft_plan(h_angles, nproj, npix, npix);
+    tc::DevBuf c(sizeof(float2) * (size_t)nslices * nproj * npix);
+    nu.type2_real(d_x, c.as<float2>(), nslices, s);
+    const double n3 = (double)npix * npix * npix;
+    nu.type1_real(c.as<float2>(), d_g, nslices, (float)(1.0 / n3), s);
+    const long N = (long)nslices * npix * npix;
+    tc::affine_sub(d_g, d_sinoT, d_g, N, s);
+    TC_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int tc_function_value(const float* d_x, const float* d_sino, int nslices,
+                      int npix, const float* h_angles, int nproj,
+                      double* h_out, void* stream) {
+  return guarded([&] {
+    TC_REQUIRE(d_x && d_sino && h_angles && h_out, "null argument");
+    hipStream_t s = as_stream(stream);
+    const long ns = (long)nslices * nproj * npix;
+    tc::DevBuf px(sizeof(float) * ns);
+    tc::project(d_x, px.as<float>(), nslices, npix, h_angles, nproj, s);
+    *h_out = tc::diff_norm2(px.as<float>(), d_sino, ns, s);
+  });
+}
+
+int tc_xerror(const float* d_a, const float* d_b, long n, double* h_out,
+              void* stream) {
+  return guarded([&] {
+    TC_REQUIRE(d_a && d_b && h_out, "null argument");
+    *h_out = tc::diff_norm2(d_a, d_b, n, as_stream(stream));
+  });
+}
+
+int tc_preproc(const float* d_sino, float* d_out, int nslices, int nproj,
+               int nrays, float center, int* h_npad, void* stream) {
+  return guarded([&] {
+    TC_REQUIRE(nslices >= 1 && nproj >= 1 && nrays >= 1, "preproc: bad shape");
+    int npad = 0, off = 0;
+    tc::pad_amount(nrays, center, &npad, &off);
+    if (h_npad) *h_npad = npad;
+    if (d_out) {
+      TC_REQUIRE(d_sino != nullptr, "null argument");
+      tc::preproc(d_sino, d_out, nslices, nproj, nrays, npad, off,
+                  as_stream(stream));
+    }
+  });
+}
+
+int tc_postproc(const float* d_img, float* d_out, int nslices, int n,
+                int npixels, void* stream) {
+  return guarded([&] {
+    TC_REQUIRE(d_img && d_out, "null argument");
+    TC_REQUIRE(nslices >= 1 && npixels >= 1 && npixels <= n, "postproc: bad shape");
+    tc::postproc(d_img, d_out, nslices, n, npixels, as_stream(stream));
+  });
+}
+
+int tc_pad2d(const float* d_img, float* d_out, int nslices, int n,
+             int padding, void* stream) {
+  return guarded([&] {
+    TC_REQUIRE(d_img && d_out, "null argument");
+    TC_REQUIRE(nslices >= 1 && n >= 1, "pad2d: bad shape");
+    tc::pad2d(d_img, d_out, nslices, n, padding, as_stream(stream));
+  });
+}
+
+int tc_psf_create(tc_psf* out, const float* h_angles, int nproj, int npix,
+                  void* stream) {
+  return guarded([&] {
+    TC_REQUIRE(out != nullptr && h_angles != nullptr, "null argument");
+    TC_REQUIRE(nproj >= 1 && npix >= 2, "psf: bad geometry");
+    hipStream_t s = as_stream(stream);
+    const size_t n1 = 2 * (size_t)npix - 1;
+    tc::DevBuf arr(sizeof(float) * n1 * n1);
+    tc::polar_psf(h_angles, nproj, npix, arr.as<float>(), s);
+    auto h = std::make_unique<tc_psf_s>();
+    h->psf = std::make_unique<tc::Psf>(arr.as<float>(), npix, s);
+    *out = h.release();
+  });
+}
+
+int tc_psf_create_from_array(tc_psf* out, const float* d_psf, int n,
+                             void* stream) {
+  return guarded([&] {
+    TC_REQUIRE(out != nullptr && d_psf != nullptr, "null argument");
+    auto h = std::make_unique<tc_psf_s>();
+    h->psf = std::make_unique<tc::Psf>(d_psf, n, as_stream(stream));
+    *out = h.release();
+  });
+}
+
+int tc_psf_get_array(tc_psf p, float* d_psf, void* stream) {
+  return guarded([&] {
+    tc::Psf& P = get_psf(p);
+    const size_t n1 = 2 * (size_t)P.n() - 1;
+    TC_HIP(hipMemcpyAsync(d_psf, P.array(), sizeof(float) * n1 * n1,
+                          hipMemcpyDeviceToDevice, as_stream(stream)));
+  });
+}
+
+int tc_psf_destroy(tc_psf p) {
+  return guarded([&] { delete p; });
+}
+
+int tc_psf_info(tc_psf p, int* n, int* fft_size) {
+  return guarded([&] {
+    tc::Psf& P = get_psf(p);
+    if (n) *n = P.n();
+    if (fft_size) *fft_size = P.M();
+  });
+}
+
+int tc_toeplitz_apply(tc_psf p, const float* d_x, float* d_out, int nslices,
+                      double* h_xTx, void* stream) {
+  return guarded([&] {
+    tc::Psf& P = get_psf(p);
+    TC_REQUIRE(d_x && d_out, "null argument");
+    hipStream_t s = as_stream(stream);
+    if (!h_xTx) {
+      P.apply(d_x, nullptr, d_out, nslices, 0, nullptr, s);
+      return;
+    }
+    const int np = tc::toeplitz_num_partials(P, nslices);
+    tc::DevBuf parts(sizeof(double) * (np + 1));
+    P.apply(d_x, nullptr, d_out, nslices, 0, parts.as<double>(), s);
+    tc::reduce_partials(parts.as<double>(), np, parts.as<double>() + np, s);
+    TC_HIP(hipMemcpyAsync(h_xTx, parts.as<double>() + np, sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+    TC_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int tc_gradient2(tc_psf p, const float* d_x, const float* d_sinoT, float* d_g,
+                 int nslices, void* stream) {
+  return guarded([&] {
+    tc::Psf& P = get_psf(p);
+    TC_REQUIRE(d_x && d_sinoT && d_g, "null argument");
+    P.apply(d_x, d_sinoT, d_g, nslices, 1, nullptr, as_stream(stream));
+  });
+}
+
+int tc_function_value2(tc_psf p, const float* d_x, const float* d_sinoT,
+                       int nslices, double sino_sq, double* h_out,
+                       void* stream) {
+  return guarded([&] {
+    tc::Psf& P = get_psf(p);
+    TC_REQUIRE(d_x && d_sinoT && h_out, "null argument");
+    hipStream_t s = as_stream(stream);
+    const long N = (long)nslices * P.n() * P.n();
+    tc::DevBuf tx(sizeof(float) * N);
+    const int np = tc::toeplitz_num_partials(P, nslices);
+    tc::DevBuf parts(sizeof(double) * (np + 1));
+    P.apply(d_x, nullptr, tx.as<float>(), nslices, 0, parts.as<double>(), s);
+    tc::reduce_partials(parts.as<double>(), np, parts.as<double>() + np, s);
+    double xtx = 0.0;
+    TC_HIP(hipMemcpyAsync(&xtx, parts.as<double>() + np, sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+    const double xb = tc::dot(d_x, d_sinoT, N, s);  // synchronises
+    *h_out = sino_sq + xtx - 2.0 * xb;
+  });
+}
+
+int tc_add_total_var2(const float* d_x, float* d_g, int nslices, int nrows,
+                      int ncols, float sigma, float p, void* stream) {
+  return guarded([&] {
+    TC_REQUIRE(d_x && d_g, "null argument");
+    tc::add_total_var(d_x, d_g, nslices, nrows, ncols, sigma, p,
+                      as_stream(stream));
+  });
+}
+
+int tc_add_total_var(const float* d_x, float* d_g, int nslices, int nrows,
+                     int ncols, float sigma, float p, void* stream) {
+  return tc_add_total_var2(d_x, d_g, nslices, nrows, ncols, sigma, p, stream);
+}
+
+int tc_add_tv_hessian(float* h_g, long n, float sigma, float* h_value) {
+  return guarded([&] {
+    (void)h_g;
+    (void)n;
+    TC_REQUIRE(sigma > 0.f, "add_tv_hessian: sigma must be > 0");
+    if (h_value) *h_value = tc::tv_hessian_zero(sigma);
+  });
+}
+
+int tc_run2(tc_psf p, const float* d_x0, const float* d_sinoT, float* d_out,
+            int nslices, int num_iters, double step, float sigma, float p_exp,
+            double sino_sq, int refresh_every, int verbose, double* h_err,
+            double* h_xerr, int* h_stats, void* stream) {
+  return guarded([&] {
+    tc::Psf& P = get_psf(p);
+    TC_REQUIRE(d_x0 && d_sinoT && d_out, "null argument");
+    tc::Run2Options o;
+    o.num_iters = num_iters;
+    o.step = step;
+    o.sigma = sigma;
+    o.p = p_exp;
+    o.sino_norm = sino_sq;
+    o.refresh_every = refresh_every;
+    o.verbose = verbose;
+    if (h_err || h_xerr) {
+      o.on_iter = [&](int it, double e, double xe, double, int) {
+        if (h_err) h_err[it] = e;
+        if (h_xerr) h_xerr[it] = xe;
+      };
+    }
+    tc::Run2Stats st =
+        tc::run2_device(P, d_x0, d_sinoT, d_out, nslices, o, as_stream(stream));
+    if (h_stats) {
+      h_stats[0] = st.iterations;
+      h_stats[1] = st.attempts;
+      h_stats[2] = st.backtracks;
+      h_stats[3] = st.toeplitz_applies;
+    }
+  });
+}
+
+static void fill_info(const tc::Mbir2Result& r, double* h_info) {
+  if (!h_info) return;
+  h_info[0] = r.lipschitz;
+  h_info[1] = r.step;
+  h_info[2] = r.sino_norm;
+  h_info[3] = r.max_sino;
+  h_info[4] = r.padded;
+  h_info[5] = r.stats.iterations;
+}
+
+int tc_mbir2(const float* d_sino, float* d_recon, int nslices, int nproj,
+             int nrays, const float* h_angles, float center, int num_iters,
+             float sigma, float tol, float xtol, int refresh_every,
+             int verbose, double* h_info, void* stream) {
+  return guarded([&] {
+    (void)tol;
+    (void)xtol;
+    TC_REQUIRE(d_sino && d_recon && h_angles, "null argument");
+    TC_REQUIRE(num_iters >= 0, "mbir: num_iters must be >= 0");
+    tc::Mbir2Options o;
+    o.num_iters = num_iters;
+    o.sigma = sigma;
+    o.refresh_every = refresh_every;
+    o.verbose = verbose;
+    tc::Mbir2Result r = tc::mbir2(d_sino, d_recon, nslices, nproj, nrays,
+                                  h_angles, center, o, as_stream(stream));
+    fill_info(r, h_info);
+  });
+}
+
+int tc_mbir2_host(const float* h_sino, float* h_recon, int nslices, int nproj,
+                  int nrays, const float* h_angles, float center,
+                  int num_iters, float sigma, float tol, float xtol,
+                  int verbose) {
+  return guarded([&] {
+    (void)tol;
+    (void)xtol;
+    TC_REQUIRE(h_sino && h_recon && h_angles, "null argument");
+    TC_REQUIRE(nslices >= 1 && nproj >= 1 && nrays >= 2, "mbir: bad shape");
+    const size_t ns = (size_t)nslices * nproj * nrays;
+    const size_t nr = (size_t)nslices * nrays * nrays;
+    tc::DevBuf ds(sizeof(float) * ns), dr(sizeof(float) * nr);
+    TC_HIP(hipMemcpy(ds.as<void>(), h_sino, sizeof(float) * ns,
+                     hipMemcpyHostToDevice));
+    tc::Mbir2Options o;
+    o.num_iters = num_iters;
+    o.sigma = sigma;
+    o.verbose = verbose;
+    tc::mbir2(ds.as<float>(), dr.as<float>(), nslices, nproj, nrays, h_angles,
+              center, o, nullptr);
+    TC_HIP(hipMemcpy(h_recon, dr.as<void>(), sizeof(float) * nr,
+                     hipMemcpyDeviceToHost));
+  });
+}
+
+int tc_ktimer_enable(int kernel_class, int on) {
+  return guarded([&] { tc::ktimer_enable(kernel_class, on != 0); });
+}
+
+int tc_ktimer_collect(int kernel_class, double* total_ms, long* count) {
+  return guarded([&] {
+    double t = 0.0;
+    long c = 0;
+    tc::ktimer_collect(kernel_class, &t, &c);
+    if (total_ms) *total_ms = t;
+    if (count) *count = c;
+  });
+}
+
+}  // extern "C"

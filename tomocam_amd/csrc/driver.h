@@ -1,0 +1,49 @@
+// Pre/post-processing and the full mbir2 pipeline (device resident).
+#pragma once
+
+#include "mbir.h"
+
+namespace tc {
+
+// src/preproc.cpp:80-85: padding per side and centre offset.
+void pad_amount(int nrays, float center, int* npad, int* offset);
+
+// src/preproc.cpp:62-65,73-108: out[s][p][z] = padded[(z + off) mod N'],
+// padded = in zero-padded by npad on both sides.  in [S][P][N] -> [S][P][N+2npad].
+void preproc(const float* in, float* out, int S, int P, int N, int npad,
+             int offset, hipStream_t stream);
+
+// src/postproc.cpp:69-102: crop [npad, npad + n) on both axes.
+void postproc(const float* in, float* out, int S, int nin, int nout,
+              hipStream_t stream);
+
+// src/padding.cpp:37-91 / src/gpu/padding.cu:119-150: symmetric zero pad by
+// `padding` in total (shift padding/2) on both axes.
+void pad2d(const float* in, float* out, int S, int nin, int padding,
+           hipStream_t stream);
+
+struct Mbir2Options {
+  int num_iters = 100;
+  float sigma = 100.f;
+  float p = 1.2f;
+  int refresh_every = 10;
+  int verbose = 0;
+};
+
+struct Mbir2Result {
+  Run2Stats stats;
+  double lipschitz = 0.0;
+  double step = 0.0;
+  double sino_norm = 0.0;
+  double max_sino = 0.0;
+  int npad = 0;
+  int padded = 0;
+};
+
+// tomocam::mbir2 (src/mbir2.cpp:40-145) without an initial guess:
+// sino [S][P][N] (device) -> recon [S][N][N] (device).
+Mbir2Result mbir2(const float* sino, float* recon, int S, int P, int N,
+                  const float* h_angles, float center, const Mbir2Options& o,
+                  hipStream_t stream);
+
+}  // namespace tc

@@ -1,0 +1,178 @@
+// Pre/post-processing kernels and the device-resident mbir2 pipeline.
+#include <cmath>
+#include <cstdio>
+
+#include "driver.h"
+#include "nufft.h"
+#include "toeplitz.h"
+#include "vecops.h"
+
+namespace tc {
+
+void pad_amount(int nrays, float center, int* npad, int* offset) {
+  const int cen = (int)std::round(center);  // halves away from zero
+  const int off = (nrays + 1) / 2 - cen;
+  int np = static_cast<int>(0.42 * nrays) / 2;
+  if (std::abs(off) > np) np = std::abs(off);
+  *npad = np;
+  *offset = off;
+}
+
+__global__ void preproc_kernel(const float* __restrict__ in, float* out,
+                               long rows, int N, int N2, int npad, int off) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= rows * N2) return;
+  const long r = idx / N2;
+  const int z = (int)(idx - r * N2);
+  int zs = (z + off) % N2;
+  if (zs < 0) zs += N2;
+  const int zi = zs - npad;
+  out[idx] = (zi >= 0 && zi < N) ? in[r * N + zi] : 0.f;
+}
+
+void preproc(const float* in, float* out, int S, int P, int N, int npad,
+             int offset, hipStream_t stream) {
+  const int N2 = N + 2 * npad;
+  const long rows = (long)S * P;
+  hipLaunchKernelGGL(preproc_kernel, dim3(cdiv(rows * N2, 256)), dim3(256), 0,
+                     stream, in, out, rows, N, N2, npad, offset);
+  TC_HIP(hipGetLastError());
+}
+
+__global__ void crop_kernel(const float* __restrict__ in, float* out, int S,
+                            int nin, int nout, int shift) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long per = (long)nout * nout;
+  if (idx >= per * S) return;
+  const long s = idx / per;
+  const int rem = (int)(idx - s * per);
+  const int r = rem / nout, c = rem - (rem / nout) * nout;
+  out[idx] = in[(s * nin + r + shift) * nin + c + shift];
+}
+
+void postproc(const float* in, float* out, int S, int nin, int nout,
+              hipStream_t stream) {
+  TC_REQUIRE(nout <= nin, "postproc: output larger than input");
+  const int shift = (nin - nout) / 2;
+  const long tot = (long)S * nout * nout;
+  hipLaunchKernelGGL(crop_kernel, dim3(cdiv(tot, 256)), dim3(256), 0, stream,
+                     in, out, S, nin, nout, shift);
+  TC_HIP(hipGetLastError());
+}
+
+__global__ void pad2d_kernel(const float* __restrict__ in, float* out, int S,
+                             int nin, int nout, int shift) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long per = (long)nout * nout;
+  if (idx >= per * S) return;
+  const long s = idx / per;
+  const int rem = (int)(idx - s * per);
+  const int r = rem / nout - shift, c = rem - (rem / nout) * nout - shift;
+  out[idx] = (r >= 0 && r < nin && c >= 0 && c < nin)
+                 ? in[(s * nin + r) * nin + c]
+                 : 0.f;
+}
+
+void pad2d(const float* in, float* out, int S, int nin, int padding,
+           hipStream_t stream) {
+  const int nout = nin + std::abs(padding);
+  const int shift = std::abs(padding) / 2;
+  const long tot = (long)S * nout * nout;
+  hipLaunchKernelGGL(pad2d_kernel, dim3(cdiv(tot, 256)), dim3(256), 0, stream,
+                     in, out, S, nin, nout, shift);
+  TC_HIP(hipGetLastError());
+}
+
+__global__ void fill_kernel(float* a, long n, float v) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+Mbir2Result mbir2(const float* sino, float* recon, int S, int P, int N,
+                  const float* h_angles, float center, const Mbir2Options& o,
+                  hipStream_t stream) {
+  TC_REQUIRE(S >= 1 && P >= 1 && N >= 2, "mbir2: bad sinogram shape");
+  TC_REQUIRE(h_angles != nullptr, "mbir2: null angles");
+  Mbir2Result res;
+  const long nsino = (long)S * P * N;
+
+  // normalise (src/mbir2.cpp:46-54)
+  float mx = 0.f, mn = 0.f;
+  minmax(sino, nsino, &mx, &mn, stream);
+  res.max_sino = mx;
+  if (mx == 0.f)
+    throw Error(TC_ERR_VALUE, "mbir2: sinogram has ZERO max value");
+  DevBuf bsino2(sizeof(float) * nsino);
+  affine(sino, bsino2.as<float>(), nsino, 0.f, 1.0f / mx, stream);
+
+  // FBP initial guess at the unpadded width (src/mbir2.cpp:60)
+  DevBuf bx0(sizeof(float) * (size_t)S * N * N);
+  backproject(bsino2.as<float>(), bx0.as<float>(), S, P, N, h_angles, true,
+              stream);
+
+  // pad and centre (src/mbir2.cpp:63-66)
+  int npad = 0, off = 0;
+  pad_amount(N, center, &npad, &off);
+  const int n2 = N + 2 * npad;
+  res.npad = npad;
+  res.padded = n2;
+  DevBuf bsinop(sizeof(float) * (size_t)S * P * n2);
+  preproc(bsino2.as<float>(), bsinop.as<float>(), S, P, N, npad, off, stream);
+  bsino2.release();
+  DevBuf bx0p(sizeof(float) * (size_t)S * n2 * n2);
+  pad2d(bx0.as<float>(), bx0p.as<float>(), S, N, n2 - N, stream);
+  bx0.release();
+
+  // b = P^T y on the padded geometry, ||y||^2 (src/mbir2.cpp:74-77)
+  DevBuf bb(sizeof(float) * (size_t)S * n2 * n2);
+  backproject(bsinop.as<float>(), bb.as<float>(), S, P, n2, h_angles, false,
+              stream);
+  res.sino_norm = dot(bsinop.as<float>(), bsinop.as<float>(),
+                      (long)S * P * n2, stream);
+  bsinop.release();
+
+  // PSF and its spectrum (src/mbir2.cpp:86-97)
+  const size_t n1 = 2 * (size_t)n2 - 1;
+  DevBuf bpsf(sizeof(float) * n1 * n1);
+  polar_psf(h_angles, P, n2, bpsf.as<float>(), stream);
+  Psf psf(bpsf.as<float>(), n2, stream);
+  bpsf.release();
+
+  // L = max(Re(A^H A 1)/n^3) = max(T 1); the qGGMRF Hessian term never
+  // reaches the host in the reference (src/mbir2.cpp:101-112,
+  // src/gpu/tv_hessian.cu:53-65).
+  {
+    const long nn = (long)n2 * n2;
+    DevBuf ones(sizeof(float) * nn), t1(sizeof(float) * nn);
+    hipLaunchKernelGGL(fill_kernel, dim3(cdiv(nn, 256)), dim3(256), 0, stream,
+                       ones.as<float>(), nn, 1.0f);
+    TC_HIP(hipGetLastError());
+    psf.apply(ones.as<float>(), nullptr, t1.as<float>(), 1, 0, nullptr, stream);
+    float lmax = 0.f, lmin = 0.f;
+    minmax(t1.as<float>(), nn, &lmax, &lmin, stream);
+    res.lipschitz = lmax;
+  }
+  double step = 1.0 / res.lipschitz;
+  if (step > 1.0) step = 1.0;
+  res.step = step;
+
+  // Optimizer::run2 (src/mbir2.cpp:139-143)
+  Run2Options ro;
+  ro.num_iters = o.num_iters;
+  ro.step = step;
+  ro.sigma = o.sigma;
+  ro.p = o.p;
+  ro.sino_norm = res.sino_norm;
+  ro.refresh_every = o.refresh_every;
+  ro.verbose = o.verbose;
+  DevBuf bsol(sizeof(float) * (size_t)S * n2 * n2);
+  res.stats = run2_device(psf, bx0p.as<float>(), bb.as<float>(),
+                          bsol.as<float>(), S, ro, stream);
+
+  // crop (src/mbir2.cpp:144)
+  postproc(bsol.as<float>(), recon, S, n2, N, stream);
+  TC_HIP(hipStreamSynchronize(stream));
+  return res;
+}
+
+}  // namespace tc

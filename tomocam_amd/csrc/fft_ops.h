@@ -1,0 +1,31 @@
+// Host-callable FFT operations built on the LDS Stockham kernels.
+#pragma once
+
+#include "tc_common.h"
+
+namespace tc {
+
+struct LineFFTArgs {
+  const float2* in = nullptr;
+  float2* out = nullptr;  // may equal `in`
+  int L = 0;               // transform length (2,3,5,7-smooth)
+  int lines_per_batch = 0;
+  int nbatch = 1;
+  long bstride = 0;   // complex elements between batches
+  long lstride = 0;   // complex elements between lines of a batch
+  long estride = 1;   // complex elements between elements of a line
+  int nin = 0;        // leading input elements read (rest treated as zero)
+  int nout = 0;       // leading output elements written
+  bool colmajor = false;  // lines adjacent in memory (column transforms)
+  float scale = 1.0f;
+};
+
+// dir = -1: exp(-2 pi i k n / L) (forward); +1: exp(+2 pi i k n / L); both
+// unnormalised (times `scale`).
+void fft_lines(const LineFFTArgs& a, int dir, hipStream_t stream);
+
+// In-place unnormalised 2D transform of `nbatch` complex [rows][cols] arrays.
+void fft2d(float2* data, int nbatch, int rows, int cols, long row_pitch,
+           long slice_pitch, int dir, hipStream_t stream);
+
+}  // namespace tc

@@ -1,0 +1,47 @@
+// qGGMRF regulariser and the device-resident accelerated-gradient MBIR loop.
+#pragma once
+
+#include <functional>
+
+#include "tc_common.h"
+#include "toeplitz.h"
+
+namespace tc {
+
+// g[v] += sum_d w_d rho'(x_v - x_clamp(v+d)) over [S][R][C] (q = 2, c = 1e-3).
+// Reference: add_total_var2 (src/tv_update.cpp:41-106), qGGMRF_kernel
+// (src/gpu/cuTV2.cu:32-93).
+void add_total_var(const float* x, float* g, int S, int R, int C, float sigma,
+                   float p, hipStream_t stream);
+
+// sum_d w_d q / (sigma^q c) (src/gpu/potential_function.cuh:111-114).
+float tv_hessian_zero(float sigma);
+
+struct Run2Options {
+  int num_iters = 100;
+  double step = 1.0;
+  float sigma = 1.0f;
+  float p = 1.2f;
+  double sino_norm = 0.0;  // ||y||^2 of the (normalised, padded) sinogram
+  int refresh_every = 10;  // recompute T(sol), T(x) every k accepted steps
+  int verbose = 0;         // print the reference's per-iteration line
+  // optional per-iteration record: (iter, error, x-err, step, backtracks)
+  std::function<void(int, double, double, double, int)> on_iter;
+};
+
+struct Run2Stats {
+  int iterations = 0;
+  int attempts = 0;
+  int backtracks = 0;
+  int toeplitz_applies = 0;
+  double last_error = 0.0;
+  double last_xerr = 0.0;
+};
+
+// Optimizer::run2 (src/optimize.h:96-154) on device-resident state.
+// x0: initial guess [S][n][n] (device, read); b: P^T y [S][n][n] (device);
+// out: solution [S][n][n] (device, written).
+Run2Stats run2_device(Psf& psf, const float* x0, const float* b, float* out,
+                      int S, const Run2Options& opt, hipStream_t stream);
+
+}  // namespace tc

@@ -1,0 +1,607 @@
+// Polar-grid NUFFT (tile-owned LDS spreading, gather interpolation),
+// Bluestein ray DFT and the projectors built on them.
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
+
+#include "fft_device.h"
+#include "fft_ops.h"
+#include "lds_launch.h"
+#include "nufft.h"
+
+namespace tc {
+
+// ---------------------------------------------------------------------------
+// kernel function and host-side correction factors
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float es_kernel(float z, float beta) {
+  const float s = 1.0f - z * z;
+  return s >= 0.f ? __expf(beta * (sqrtf(s) - 1.0f)) : 0.f;
+}
+
+// Gauss-Legendre nodes/weights on [-1, 1] (Newton iteration on P_n).
+static void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(n, 0.0);
+  w.assign(n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    double z = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+    double pp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double p1 = 1.0, p2 = 0.0;
+      for (int j = 1; j <= n; ++j) {
+        double p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1.0);
+      double z1 = z;
+      z = z1 - p1 / pp;
+      if (std::fabs(z - z1) < 1e-15) {
+        p1 = 1.0;
+        p2 = 0.0;
+        for (int j = 1; j <= n; ++j) {
+          double p3 = p2;
+          p2 = p1;
+          p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+        }
+        pp = n * (z * p1 - p2) / (z * z - 1.0);
+        break;
+      }
+    }
+    x[i] = z;
+    w[i] = 2.0 / ((1.0 - z * z) * pp * pp);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// spreading (type 1): each workgroup owns a T x T tile of the fine grid and
+// gathers every ray segment whose kernel footprint meets it (rays are
+// enumerated analytically, so no sorting and no global atomics).  A wave
+// processes one point at a time, lane (a, b) owning footprint cell (a, b).
+// ---------------------------------------------------------------------------
+
+constexpr int SP_T = 32;
+constexpr int SP_SB = 4;
+constexpr int SP_LIST = 2048;
+
+struct SpreadArgs {
+  const float2* c;  // [nsl][P][N]
+  float2* fine;     // [nsl][nf][nf]
+  const double* ax;
+  const double* ay;
+  int P, N, cidx, nf, nsl, ntx;
+  double umax;
+  float beta;
+};
+
+// { t : lo < U0 + t a <= hi } intersected into [tlo, thi] (superset bounds)
+__device__ __forceinline__ void ray_interval(double a, double U0, double lo,
+                                             double hi, double& tlo,
+                                             double& thi) {
+  if (fabs(a) < 1e-12) {
+    if (!(U0 > lo - 1e-9 && U0 <= hi + 1e-9)) {
+      tlo = 1e30;
+      thi = -1e30;
+    }
+    return;
+  }
+  double t1 = (lo - U0) / a, t2 = (hi - U0) / a;
+  if (t1 > t2) {
+    double q = t1;
+    t1 = t2;
+    t2 = q;
+  }
+  tlo = fmax(tlo, t1);
+  thi = fmin(thi, t2);
+}
+
+__global__ void __launch_bounds__(256) spread_kernel(SpreadArgs a) {
+  __shared__ float2 acc[SP_SB][SP_T * SP_T];
+  __shared__ int2 seg[SP_LIST];
+  __shared__ int nseg;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wid = tid >> 6;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int s0 = blockIdx.y * SP_SB;
+  const int ns = min(SP_SB, a.nsl - s0);
+  const int ty = tile / a.ntx;
+  const int tx = tile - ty * a.ntx;
+  const int X0 = tx * SP_T, Y0 = ty * SP_T;
+  const int X1 = min(X0 + SP_T, a.nf), Y1 = min(Y0 + SP_T, a.nf);
+  for (int i = tid; i < SP_SB * SP_T * SP_T; i += 256)
+    (&acc[0][0])[i] = make_float2(0.f, 0.f);
+  const double hw = 0.5 * NUFFT_W;
+  const double lox = X0 - hw, hix = X1 - 1 + hw;
+  const double loy = Y0 - hw, hiy = Y1 - 1 + hw;
+  const int la = lane >> 3, lb = lane & 7;
+  const long sstride = (long)a.P * a.N;
+
+  for (int ky = -1; ky <= 1; ++ky) {
+    for (int kx = -1; kx <= 1; ++kx) {
+      const double U0 = 0.5 * a.nf + kx * (double)a.nf;
+      const double V0 = 0.5 * a.nf + ky * (double)a.nf;
+      if (U0 + a.umax <= lox || U0 - a.umax > hix || V0 + a.umax <= loy ||
+          V0 - a.umax > hiy)
+        continue;  // uniform over the workgroup
+      __syncthreads();
+      if (tid == 0) nseg = 0;
+      __syncthreads();
+      for (int p = tid; p < a.P; p += 256) {
+        double tlo = -1e30, thi = 1e30;
+        ray_interval(a.ax[p], U0, lox, hix, tlo, thi);
+        ray_interval(a.ay[p], V0, loy, hiy, tlo, thi);
+        if (tlo <= thi) {
+          tlo = fmax(tlo, -(double)a.N - 2.0);
+          thi = fmin(thi, (double)a.N + 2.0);
+          int jlo = max(0, (int)floor(tlo) + a.cidx - 1);
+          int jhi = min(a.N - 1, (int)ceil(thi) + a.cidx + 1);
+          if (jlo <= jhi) {
+            int k = atomicAdd(&nseg, 1);
+            if (k < SP_LIST) seg[k] = make_int2(p, jlo | ((jhi - jlo) << 16));
+          }
+        }
+      }
+      __syncthreads();
+      const int nsg = min(nseg, SP_LIST);
+      for (int e = wid; e < nsg; e += 4) {
+        const int2 sg = seg[e];
+        const int p = sg.x;
+        const int jlo = sg.y & 0xffff;
+        const int jend = jlo + (sg.y >> 16) + 1;
+        const double axp = a.ax[p], ayp = a.ay[p];
+        const float2* cp = a.c + (long)p * a.N;
+        for (int j = jlo; j < jend; ++j) {
+          const double t = (double)(j - a.cidx);
+          const double u = U0 + t * axp;
+          const double v = V0 + t * ayp;
+          const int cx = (int)ceil(u - hw) + lb;
+          const int cy = (int)ceil(v - hw) + la;
+          if (cx >= X0 && cx < X1 && cy >= Y0 && cy < Y1) {
+            const float zx = (float)((cx - u) * (2.0 / NUFFT_W));
+            const float zy = (float)((cy - v) * (2.0 / NUFFT_W));
+            const float wgt = es_kernel(zx, a.beta) * es_kernel(zy, a.beta);
+            const int cell = (cy - Y0) * SP_T + (cx - X0);
+            for (int s = 0; s < ns; ++s) {
+              const float2 cv = cp[(long)(s0 + s) * sstride + j];
+              atomicAdd(&acc[s][cell].x, cv.x * wgt);
+              atomicAdd(&acc[s][cell].y, cv.y * wgt);
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < ns * SP_T * SP_T; i += 256) {
+    const int s = i / (SP_T * SP_T);
+    const int cell = i - s * SP_T * SP_T;
+    const int ly = cell / SP_T, lx = cell - ly * SP_T;
+    if (Y0 + ly < Y1 && X0 + lx < X1)
+      a.fine[((long)(s0 + s) * a.nf + Y0 + ly) * a.nf + X0 + lx] = acc[s][cell];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// interpolation (type 2): one thread per polar point, all slices of a batch
+// ---------------------------------------------------------------------------
+
+struct InterpArgs {
+  const float2* fine;  // [nsl][nf][nf]
+  float2* c;           // [nsl][P][N]
+  const double* ax;
+  const double* ay;
+  int P, N, cidx, nf, nsl;
+  float beta;
+};
+
+__global__ void __launch_bounds__(256) interp_kernel(InterpArgs a) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)a.P * a.N) return;
+  const int p = (int)(idx / a.N);
+  const int j = (int)(idx - (long)p * a.N);
+  const double hw = 0.5 * NUFFT_W;
+  const double t = (double)(j - a.cidx);
+  const double u = 0.5 * a.nf + t * a.ax[p];
+  const double v = 0.5 * a.nf + t * a.ay[p];
+  const int l0x = (int)ceil(u - hw);
+  const int l0y = (int)ceil(v - hw);
+  float wx[NUFFT_W], wy[NUFFT_W];
+  int ix[NUFFT_W], iy[NUFFT_W];
+#pragma unroll
+  for (int b = 0; b < NUFFT_W; ++b) {
+    wx[b] = es_kernel((float)((l0x + b - u) * (2.0 / NUFFT_W)), a.beta);
+    wy[b] = es_kernel((float)((l0y + b - v) * (2.0 / NUFFT_W)), a.beta);
+    int qx = l0x + b, qy = l0y + b;
+    qx += qx < 0 ? a.nf : (qx >= a.nf ? -a.nf : 0);
+    qy += qy < 0 ? a.nf : (qy >= a.nf ? -a.nf : 0);
+    ix[b] = qx;
+    iy[b] = qy;
+  }
+  for (int s = 0; s < a.nsl; ++s) {
+    const float2* g = a.fine + (long)s * a.nf * a.nf;
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int r = 0; r < NUFFT_W; ++r) {
+      const float2* row = g + (long)iy[r] * a.nf;
+      float2 ra = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int b = 0; b < NUFFT_W; ++b) {
+        const float2 q = row[ix[b]];
+        ra.x = fmaf(q.x, wx[b], ra.x);
+        ra.y = fmaf(q.y, wx[b], ra.y);
+      }
+      acc.x = fmaf(ra.x, wy[r], acc.x);
+      acc.y = fmaf(ra.y, wy[r], acc.y);
+    }
+    a.c[(long)s * a.P * a.N + idx] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// deconvolution: fine-grid spectrum <-> modes k in [-floor(nm/2), ceil(nm/2))
+// ---------------------------------------------------------------------------
+
+__global__ void deconv_real_kernel(const float2* __restrict__ fine, float* out,
+                                   const float* __restrict__ corr, int nm,
+                                   int nf, int nsl, float scale) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long per = (long)nm * nm;
+  if (idx >= per * nsl) return;
+  const int s = (int)(idx / per);
+  const int rem = (int)(idx - (long)s * per);
+  const int k2i = rem / nm, k1i = rem - (rem / nm) * nm;
+  const int h = nm / 2;
+  int i1 = k1i - h, i2 = k2i - h;
+  i1 += i1 < 0 ? nf : 0;
+  i2 += i2 < 0 ? nf : 0;
+  const float2 v = fine[((long)s * nf + i2) * nf + i1];
+  out[idx] = v.x * corr[k1i] * corr[k2i] * scale;
+}
+
+__global__ void precorrect_kernel(const float* __restrict__ f, float2* fine,
+                                  const float* __restrict__ corr, int nm,
+                                  int nf, int nsl) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long per = (long)nf * nf;
+  if (idx >= per * nsl) return;
+  const int s = (int)(idx / per);
+  const long rem = idx - (long)s * per;
+  const int i2 = (int)(rem / nf), i1 = (int)(rem - (long)i2 * nf);
+  const int h = nm / 2, up = (nm + 1) / 2;
+  int k1 = i1 < up ? i1 : (i1 >= nf - h ? i1 - nf : nf);
+  int k2 = i2 < up ? i2 : (i2 >= nf - h ? i2 - nf : nf);
+  float2 v = make_float2(0.f, 0.f);
+  if (k1 != nf && k2 != nf) {
+    const int k1i = k1 + h, k2i = k2 + h;
+    v.x = f[((long)s * nm + k2i) * nm + k1i] * corr[k1i] * corr[k2i];
+  }
+  fine[idx] = v;
+}
+
+// ---------------------------------------------------------------------------
+// PolarNufft
+// ---------------------------------------------------------------------------
+
+PolarNufft::PolarNufft(const float* h_angles, int nproj, int npix, int nmodes)
+    : P_(nproj), N_(npix), nm_(nmodes) {
+  TC_REQUIRE(nproj >= 1 && npix >= 2 && nmodes >= 1, "NUFFT: bad geometry");
+  TC_REQUIRE(nproj <= SP_LIST, "NUFFT: too many projection angles");
+  TC_REQUIRE(npix < 65536, "NUFFT: too many samples per ray");
+  nf_ = fft_next_smooth(std::max(2 * nmodes, 2 * NUFFT_W), true);
+  beta_ = (float)(2.30 * NUFFT_W);
+  std::vector<double> hx(P_), hy(P_);
+  const double sc = (double)nf_ / (double)(npix - 1);
+  for (int p = 0; p < P_; ++p) {
+    const double th = (double)h_angles[p];
+    hx[p] = std::cos(th) * sc;
+    hy[p] = std::sin(th) * sc;
+  }
+  ax_.alloc(sizeof(double) * P_);
+  ay_.alloc(sizeof(double) * P_);
+  TC_HIP(hipMemcpy(ax_.as<void>(), hx.data(), sizeof(double) * P_,
+                   hipMemcpyHostToDevice));
+  TC_HIP(hipMemcpy(ay_.as<void>(), hy.data(), sizeof(double) * P_,
+                   hipMemcpyHostToDevice));
+  // corr(k) = (-1)^k h / psi_hat(k) = (-1)^k 2 / (w int phi(z) cos(k z w h/2))
+  std::vector<double> zq, wq;
+  gauss_legendre(120, zq, wq);
+  const double h = 2.0 * M_PI / nf_;
+  const double beta = (double)beta_;
+  std::vector<float> corr(nm_);
+  for (int ki = 0; ki < nm_; ++ki) {
+    const int k = ki - nm_ / 2;
+    double integ = 0.0;
+    for (size_t q = 0; q < zq.size(); ++q) {
+      const double z = zq[q];
+      const double phi = std::exp(beta * (std::sqrt(1.0 - z * z) - 1.0));
+      integ += wq[q] * phi * std::cos(k * z * NUFFT_W * h / 2.0);
+    }
+    double c = 2.0 / (NUFFT_W * integ);
+    if (k & 1) c = -c;
+    corr[ki] = (float)c;
+  }
+  corr_.alloc(sizeof(float) * nm_);
+  TC_HIP(hipMemcpy(corr_.as<void>(), corr.data(), sizeof(float) * nm_,
+                   hipMemcpyHostToDevice));
+}
+
+float2* PolarNufft::fine(int batch) {
+  if (batch > fine_batch_) {
+    fine_.alloc(sizeof(float2) * (size_t)batch * nf_ * nf_);
+    fine_batch_ = batch;
+  }
+  return fine_.as<float2>();
+}
+
+static int fine_batch(int nf, int S) {
+  const double per = 8.0 * nf * (double)nf;
+  int b = (int)std::max(1.0, std::floor(1.0e9 / per));
+  return std::max(1, std::min(S, b));
+}
+
+void PolarNufft::type1_real(const float2* c, float* out, int S, float scale,
+                            hipStream_t stream) {
+  TC_REQUIRE(S >= 1, "NUFFT: need at least one slice");
+  const int B = fine_batch(nf_, S);
+  float2* fg = fine(B);
+  const int ntx = (int)cdiv(nf_, SP_T);
+  const int cidx = N_ / 2;
+  const double umax =
+      std::max(cidx, N_ - 1 - cidx) * ((double)nf_ / (N_ - 1)) + 1.0;
+  for (int s0 = 0; s0 < S; s0 += B) {
+    const int nb = std::min(B, S - s0);
+    SpreadArgs a;
+    a.c = c + (long)s0 * P_ * N_;
+    a.fine = fg;
+    a.ax = ax_.as<double>();
+    a.ay = ay_.as<double>();
+    a.P = P_;
+    a.N = N_;
+    a.cidx = cidx;
+    a.nf = nf_;
+    a.nsl = nb;
+    a.ntx = ntx;
+    a.umax = umax;
+    a.beta = beta_;
+    dim3 grid((unsigned)(ntx * ntx), cdiv(nb, SP_SB));
+    hipLaunchKernelGGL(spread_kernel, grid, dim3(256), 0, stream, a);
+    TC_HIP(hipGetLastError());
+    fft2d(fg, nb, nf_, nf_, nf_, (long)nf_ * nf_, +1, stream);
+    const long tot = (long)nb * nm_ * nm_;
+    hipLaunchKernelGGL(deconv_real_kernel, dim3(cdiv(tot, 256)), dim3(256), 0,
+                       stream, fg, out + (long)s0 * nm_ * nm_, corr_.as<float>(),
+                       nm_, nf_, nb, scale);
+    TC_HIP(hipGetLastError());
+  }
+}
+
+void PolarNufft::type2_real(const float* f, float2* c, int S,
+                            hipStream_t stream) {
+  TC_REQUIRE(S >= 1, "NUFFT: need at least one slice");
+  const int B = fine_batch(nf_, S);
+  float2* fg = fine(B);
+  for (int s0 = 0; s0 < S; s0 += B) {
+    const int nb = std::min(B, S - s0);
+    const long tot = (long)nb * nf_ * nf_;
+    hipLaunchKernelGGL(precorrect_kernel, dim3(cdiv(tot, 256)), dim3(256), 0,
+                       stream, f + (long)s0 * nm_ * nm_, fg, corr_.as<float>(),
+                       nm_, nf_, nb);
+    TC_HIP(hipGetLastError());
+    fft2d(fg, nb, nf_, nf_, nf_, (long)nf_ * nf_, -1, stream);
+    InterpArgs a;
+    a.fine = fg;
+    a.c = c + (long)s0 * P_ * N_;
+    a.ax = ax_.as<double>();
+    a.ay = ay_.as<double>();
+    a.P = P_;
+    a.N = N_;
+    a.cidx = N_ / 2;
+    a.nf = nf_;
+    a.nsl = nb;
+    a.beta = beta_;
+    hipLaunchKernelGGL(interp_kernel, dim3(cdiv((long)P_ * N_, 256)),
+                       dim3(256), 0, stream, a);
+    TC_HIP(hipGetLastError());
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Bluestein ray DFT
+// ---------------------------------------------------------------------------
+
+template <int NB, bool IN_REAL, bool OUT_REAL, bool RAMP>
+__global__ void __launch_bounds__(1024)
+ray_dft_kernel(const void* __restrict__ in, void* __restrict__ out,
+               const float2* __restrict__ chirp, const float2* __restrict__ bhat,
+               FFTPlan P, int N, long nrays, int lpw, float scale) {
+  extern __shared__ float2 sm[];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int L = P.L;
+  const long r0 = (long)blockIdx.x * lpw;
+  for (int t = tid; t < lpw * L; t += nthr) {
+    const int l = t / L;
+    const int z = t - l * L;
+    float2 v = make_float2(0.f, 0.f);
+    if (z < N && r0 + l < nrays) {
+      const long o = (r0 + l) * N + z;
+      float2 a = IN_REAL ? make_float2(static_cast<const float*>(in)[o], 0.f)
+                         : static_cast<const float2*>(in)[o];
+      v = cmul(a, chirp[z]);
+    }
+    sm[t] = v;
+  }
+  __syncthreads();
+  fft_lds<-1, NB>(sm, lpw, L, P, tid, nthr);
+  for (int t = tid; t < lpw * L; t += nthr) {
+    const int k = t - (t / L) * L;
+    sm[t] = cmul(sm[t], bhat[k]);
+  }
+  __syncthreads();
+  fft_lds<1, NB>(sm, lpw, L, P, tid, nthr);
+  const int cidx = N / 2;
+  for (int t = tid; t < lpw * N; t += nthr) {
+    const int l = t / N;
+    const int j = t - l * N;
+    if (r0 + l >= nrays) continue;
+    float2 v = cmul(sm[l * L + j], chirp[j]);
+    float f = scale;
+    if (RAMP) f *= (float)abs(j - cidx) / (float)N;
+    const long o = (r0 + l) * N + j;
+    if (OUT_REAL)
+      static_cast<float*>(out)[o] = v.x * f;
+    else
+      static_cast<float2*>(out)[o] = cscale(v, f);
+  }
+}
+
+RayDft::RayDft(int N, int sign) : N_(N), sign_(sign) {
+  TC_REQUIRE(N >= 1, "ray DFT: N must be >= 1");
+  L_ = fft_next_smooth(2 * N - 1, false);
+  const int c = N / 2;
+  const long twoN = 2L * N;
+  std::vector<float2> ch(N), b(L_, make_float2(0.f, 0.f));
+  for (int z = 0; z < N; ++z) {
+    const long m = z - c;
+    const double ph = sign * M_PI * (double)((m * m) % twoN) / N;
+    ch[z] = make_float2((float)std::cos(ph), (float)std::sin(ph));
+  }
+  for (long m = -(N - 1); m <= N - 1; ++m) {
+    const double ph = -sign * M_PI * (double)((m * m) % twoN) / N;
+    const long idx = m >= 0 ? m : L_ + m;
+    b[idx] = make_float2((float)(std::cos(ph) / L_), (float)(std::sin(ph) / L_));
+  }
+  chirp_.alloc(sizeof(float2) * N);
+  bhat_.alloc(sizeof(float2) * L_);
+  TC_HIP(hipMemcpy(chirp_.as<void>(), ch.data(), sizeof(float2) * N,
+                   hipMemcpyHostToDevice));
+  TC_HIP(hipMemcpy(bhat_.as<void>(), b.data(), sizeof(float2) * L_,
+                   hipMemcpyHostToDevice));
+  LineFFTArgs a;
+  a.in = bhat_.as<float2>();
+  a.out = bhat_.as<float2>();
+  a.L = L_;
+  a.lines_per_batch = 1;
+  a.nbatch = 1;
+  a.lstride = L_;
+  a.bstride = L_;
+  a.nin = L_;
+  a.nout = L_;
+  fft_lines(a, -1, nullptr);
+  TC_HIP(hipDeviceSynchronize());
+}
+
+void RayDft::apply(const void* in, bool in_real, void* out, bool out_real,
+                   long nrays, bool ramp, float scale, hipStream_t stream) {
+  if (nrays <= 0) return;
+  const FFTLength& F = fft_length(L_);
+  LdsLaunch ll = plan_lds_launch(F, 4, 80 * 1024);
+  unsigned nblk = cdiv(nrays, ll.lpw);
+#define TC_RAY(IR, OR, RP)                                                   \
+  if (in_real == IR && out_real == OR && ramp == RP) {                       \
+    TC_DISPATCH_NB(ll.nb, {                                                  \
+      auto k = ray_dft_kernel<NB, IR, OR, RP>;                               \
+      allow_lds((const void*)k, ll.smem);                                    \
+      hipLaunchKernelGGL(k, dim3(nblk), dim3(ll.nthr), ll.smem, stream, in,  \
+                         out, chirp_.as<float2>(), bhat_.as<float2>(),       \
+                         F.plan(), N_, nrays, ll.lpw, scale);                \
+    });                                                                      \
+    TC_HIP(hipGetLastError());                                               \
+    return;                                                                  \
+  }
+  TC_RAY(true, false, false)
+  TC_RAY(true, false, true)
+  TC_RAY(false, true, false)
+  TC_RAY(false, false, false)
+  TC_RAY(true, true, false)
+#undef TC_RAY
+  throw Error(TC_ERR_INTERNAL, "ray DFT: unsupported combination");
+}
+
+// ---------------------------------------------------------------------------
+// plan caches
+// ---------------------------------------------------------------------------
+
+PolarNufft& nufft_plan(const float* h_angles, int nproj, int npix,
+                       int nmodes) {
+  static std::mutex mu;
+  using Key = std::tuple<int, int, int, int, std::vector<float>>;
+  static std::map<Key, std::unique_ptr<PolarNufft>> cache;
+  int dev = 0;
+  TC_HIP(hipGetDevice(&dev));
+  std::vector<float> ang(h_angles, h_angles + nproj);
+  Key key(dev, nproj, npix, nmodes, ang);
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(key);
+  if (it == cache.end()) {
+    if (cache.size() >= 16) cache.clear();
+    it = cache
+             .emplace(key, std::make_unique<PolarNufft>(h_angles, nproj, npix,
+                                                        nmodes))
+             .first;
+  }
+  return *it->second;
+}
+
+RayDft& ray_dft_plan(int N, int sign) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int>, std::unique_ptr<RayDft>> cache;
+  int dev = 0;
+  TC_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  auto key = std::make_tuple(dev, N, sign);
+  auto it = cache.find(key);
+  if (it == cache.end())
+    it = cache.emplace(key, std::make_unique<RayDft>(N, sign)).first;
+  return *it->second;
+}
+
+// ---------------------------------------------------------------------------
+// projectors
+// ---------------------------------------------------------------------------
+
+void project(const float* img, float* sino, int S, int N, const float* h_angles,
+             int P, hipStream_t stream) {
+  TC_REQUIRE(S >= 1 && N >= 2 && P >= 1, "project: bad shape");
+  PolarNufft& nu = nufft_plan(h_angles, P, N, N);
+  RayDft& rd = ray_dft_plan(N, +1);
+  DevBuf c(sizeof(float2) * (size_t)S * P * N);
+  nu.type2_real(img, c.as<float2>(), S, stream);
+  rd.apply(c.as<float2>(), false, sino, true, (long)S * P, false,
+           1.0f / ((float)N * (float)N), stream);
+  TC_HIP(hipStreamSynchronize(stream));
+}
+
+void backproject(const float* sino, float* img, int S, int P, int N,
+                 const float* h_angles, bool fbp, hipStream_t stream) {
+  TC_REQUIRE(S >= 1 && N >= 2 && P >= 1, "backproject: bad shape");
+  PolarNufft& nu = nufft_plan(h_angles, P, N, N);
+  RayDft& rd = ray_dft_plan(N, -1);
+  DevBuf c(sizeof(float2) * (size_t)S * P * N);
+  rd.apply(sino, true, c.as<float2>(), false, (long)S * P, fbp, 1.0f, stream);
+  nu.type1_real(c.as<float2>(), img, S, 1.0f / ((float)N * (float)N), stream);
+  TC_HIP(hipStreamSynchronize(stream));
+}
+
+__global__ void fill_ones_kernel(float2* c, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) c[i] = make_float2(1.f, 0.f);
+}
+
+void polar_psf(const float* h_angles, int P, int n, float* d_psf,
+               hipStream_t stream) {
+  TC_REQUIRE(P >= 1 && n >= 2, "PSF: bad geometry");
+  PolarNufft& nu = nufft_plan(h_angles, P, n, 2 * n - 1);
+  DevBuf ones(sizeof(float2) * (size_t)P * n);
+  const long tot = (long)P * n;
+  hipLaunchKernelGGL(fill_ones_kernel, dim3(cdiv(tot, 256)), dim3(256), 0,
+                     stream, ones.as<float2>(), tot);
+  TC_HIP(hipGetLastError());
+  nu.type1_real(ones.as<float2>(), d_psf, 1, 1.0f, stream);
+  TC_HIP(hipStreamSynchronize(stream));
+}
+
+}  // namespace tc

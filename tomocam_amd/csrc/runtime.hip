@@ -1,0 +1,75 @@
+// Error strings and optional per-kernel-class event timing.
+#include <mutex>
+#include <vector>
+
+#include "tc_common.h"
+
+namespace tc {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+const char* last_error() { return g_last_error.c_str(); }
+
+// Event pairs recorded around launches of enabled kernel classes; resolved
+// (and released) by ktimer_collect.
+struct KTimer {
+  std::mutex mu;
+  bool enabled[KC_COUNT] = {};
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs[KC_COUNT];
+  hipEvent_t open[KC_COUNT] = {};
+};
+
+static KTimer& kt() {
+  static KTimer k;
+  return k;
+}
+
+void ktimer_enable(int kc, bool on) {
+  TC_REQUIRE(kc >= 0 && kc < KC_COUNT, "bad kernel class");
+  std::lock_guard<std::mutex> lk(kt().mu);
+  kt().enabled[kc] = on;
+}
+
+void ktimer_begin(int kc, hipStream_t stream) {
+  KTimer& k = kt();
+  if (!k.enabled[kc]) return;
+  hipEvent_t e;
+  TC_HIP(hipEventCreate(&e));
+  TC_HIP(hipEventRecord(e, stream));
+  k.open[kc] = e;
+}
+
+void ktimer_end(int kc, hipStream_t stream) {
+  KTimer& k = kt();
+  if (!k.enabled[kc] || !k.open[kc]) return;
+  hipEvent_t e;
+  TC_HIP(hipEventCreate(&e));
+  TC_HIP(hipEventRecord(e, stream));
+  std::lock_guard<std::mutex> lk(k.mu);
+  k.pairs[kc].emplace_back(k.open[kc], e);
+  k.open[kc] = nullptr;
+}
+
+// Total milliseconds and launch count of class kc since the last collect.
+void ktimer_collect(int kc, double* total_ms, long* count) {
+  TC_REQUIRE(kc >= 0 && kc < KC_COUNT, "bad kernel class");
+  KTimer& k = kt();
+  std::lock_guard<std::mutex> lk(k.mu);
+  double tot = 0.0;
+  long cnt = 0;
+  for (auto& pr : k.pairs[kc]) {
+    TC_HIP(hipEventSynchronize(pr.second));
+    float ms = 0.f;
+    TC_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+    tot += ms;
+    ++cnt;
+    (void)hipEventDestroy(pr.first);
+    (void)hipEventDestroy(pr.second);
+  }
+  k.pairs[kc].clear();
+  *total_ms = tot;
+  *count = cnt;
+}
+
+}  // namespace tc

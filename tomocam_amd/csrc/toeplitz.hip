@@ -1,0 +1,311 @@
+// Toeplitz normal operator: PSF spectrum set-up and the three-pass apply.
+// Reference: src/toeplitz.h:51-159 (PointSpreadFunction ctor / convolve).
+#include <cmath>
+
+#include "fft_device.h"
+#include "fft_ops.h"
+#include "lds_launch.h"
+#include "toeplitz.h"
+
+namespace tc {
+
+// ---------------------------------------------------------------------------
+// PSF spectrum
+// ---------------------------------------------------------------------------
+
+// C[i2][i1] = PSF(m1, m2) with m = i (i < n) or i - M (i > M - n), else 0.
+__global__ void psf_place_kernel(const float* __restrict__ psf, float2* c,
+                                 int n, int M) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)M * M) return;
+  int i2 = (int)(idx / M);
+  int i1 = (int)(idx - (long)i2 * M);
+  int m1 = i1 < n ? i1 : (i1 > M - n ? i1 - M : M);
+  int m2 = i2 < n ? i2 : (i2 > M - n ? i2 - M : M);
+  float v = 0.f;
+  if (m1 != M && m2 != M) {
+    const int n1 = 2 * n - 1;
+    v = psf[(long)(m2 + n - 1) * n1 + (m1 + n - 1)];
+  }
+  c[idx] = make_float2(v, 0.f);
+}
+
+// spec[k1][k2] = Re C[k2][k1] * scale, k1 < H, k2 < M
+__global__ void psf_extract_kernel(const float2* __restrict__ c, float* spec,
+                                   int M, int H, float scale) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)M * H) return;
+  int k1 = (int)(idx / M);
+  int k2 = (int)(idx - (long)k1 * M);
+  spec[idx] = c[(long)k2 * M + k1].x * scale;
+}
+
+Psf::Psf(const float* d_psf, int n, hipStream_t stream) : n_(n) {
+  TC_REQUIRE(n >= 2, "PSF: image width must be >= 2");
+  M_ = fft_next_smooth(2 * n - 1, true);
+  H_ = M_ / 2 + 1;
+  rp_ = n + (n & 1);
+  const size_t n1 = 2 * (size_t)n - 1;
+  arr_.alloc(sizeof(float) * n1 * n1);
+  TC_HIP(hipMemcpyAsync(arr_.as<void>(), d_psf, sizeof(float) * n1 * n1,
+                        hipMemcpyDeviceToDevice, stream));
+  DevBuf c(sizeof(float2) * (size_t)M_ * M_);
+  long tot = (long)M_ * M_;
+  hipLaunchKernelGGL(psf_place_kernel, dim3(cdiv(tot, 256)), dim3(256), 0,
+                     stream, d_psf, c.as<float2>(), n, M_);
+  TC_HIP(hipGetLastError());
+  fft2d(c.as<float2>(), 1, M_, M_, M_, (long)M_ * M_, -1, stream);
+  spec_.alloc(sizeof(float) * (size_t)M_ * H_);
+  const double scale = 1.0 / ((double)n * n * n * (double)M_ * M_);
+  long tot2 = (long)M_ * H_;
+  hipLaunchKernelGGL(psf_extract_kernel, dim3(cdiv(tot2, 256)), dim3(256), 0,
+                     stream, c.as<float2>(), spec_.as<float>(), M_, H_,
+                     (float)scale);
+  TC_HIP(hipGetLastError());
+  TC_HIP(hipStreamSynchronize(stream));
+}
+
+void Psf::reserve(int nslices) {
+  if ((size_t)nslices > work_slices_) {
+    work_.alloc(sizeof(float2) * (size_t)nslices * H_ * rp_);
+    work_slices_ = nslices;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pass A: pair-packed forward row FFTs -> Wt[s][k1][row]
+// ---------------------------------------------------------------------------
+
+template <int NB>
+__global__ void __launch_bounds__(1024)
+toep_rows_fwd(const float* __restrict__ x, float2* __restrict__ wt, FFTPlan P,
+              int n, int H, int rp, int lpw, int nbps) {
+  extern __shared__ float2 sm[];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int M = P.L;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int s = b / nbps;
+  const int r0 = (b - s * nbps) * 2 * lpw;
+  const float* xs = x + (long)s * n * n;
+  for (int t = tid; t < lpw * M; t += nthr) {
+    int l = t / M;
+    int c = t - l * M;
+    int ra = r0 + 2 * l;
+    float re = 0.f, im = 0.f;
+    if (c < n) {
+      if (ra < n) re = xs[(long)ra * n + c];
+      if (ra + 1 < n) im = xs[(long)(ra + 1) * n + c];
+    }
+    sm[t] = make_float2(re, im);
+  }
+  __syncthreads();
+  fft_lds<-1, NB>(sm, lpw, M, P, tid, nthr);
+  float2* ws = wt + (long)s * H * rp;
+  for (int t = tid; t < lpw * H; t += nthr) {
+    int k = t / lpw;
+    int l = t - k * lpw;
+    int ra = r0 + 2 * l;
+    if (ra < rp) {
+      float2 zk = sm[l * M + k];
+      float2 zm = cconj(sm[l * M + (k == 0 ? 0 : M - k)]);
+      float2 sp = cadd(zk, zm), df = csub(zk, zm);
+      float4 q = make_float4(0.5f * sp.x, 0.5f * sp.y, 0.5f * df.y,
+                             -0.5f * df.x);
+      *reinterpret_cast<float4*>(ws + (long)k * rp + ra) = q;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pass B: column FFT, multiply by the real spectrum, inverse column FFT
+// ---------------------------------------------------------------------------
+
+template <int NB>
+__global__ void __launch_bounds__(1024)
+toep_cols(float2* __restrict__ wt, const float* __restrict__ spec, FFTPlan P,
+          int n, int H, int rp, int lpw, int nbps, double* partials) {
+  extern __shared__ float2 sm[];
+  __shared__ double red[16];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int M = P.L;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int s = b / nbps;
+  const int k0 = (b - s * nbps) * lpw;
+  float2* ws = wt + (long)s * H * rp;
+  for (int t = tid; t < lpw * M; t += nthr) {
+    int l = t / M;
+    int r = t - l * M;
+    float2 v = make_float2(0.f, 0.f);
+    if (r < n && k0 + l < H) v = ws[(long)(k0 + l) * rp + r];
+    sm[t] = v;
+  }
+  __syncthreads();
+  fft_lds<-1, NB>(sm, lpw, M, P, tid, nthr);
+  double acc = 0.0;
+  for (int t = tid; t < lpw * M; t += nthr) {
+    int l = t / M;
+    int k2 = t - l * M;
+    int k1 = k0 + l;
+    if (k1 < H) {
+      float h = spec[(long)k1 * M + k2];
+      float2 v = sm[t];
+      float wgt = (k1 == 0 || 2 * k1 == M) ? 1.f : 2.f;
+      acc += (double)(wgt * h * (v.x * v.x + v.y * v.y));
+      sm[t] = cscale(v, h);
+    }
+  }
+  __syncthreads();
+  fft_lds<1, NB>(sm, lpw, M, P, tid, nthr);
+  for (int t = tid; t < lpw * M; t += nthr) {
+    int l = t / M;
+    int r = t - l * M;
+    if (r < n && k0 + l < H) ws[(long)(k0 + l) * rp + r] = sm[t];
+  }
+  if (partials) {
+    double tot = block_sum(acc, red);
+    if (tid == 0) partials[b] = tot;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pass C: pair-packed inverse row FFTs, crop, epilogue
+// ---------------------------------------------------------------------------
+
+template <int NB, int MODE>
+__global__ void __launch_bounds__(1024)
+toep_rows_inv(const float2* __restrict__ wt, const float* __restrict__ bvec,
+              float* __restrict__ out, FFTPlan P, int n, int H, int rp,
+              int lpw, int nbps) {
+  extern __shared__ float2 sm[];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int M = P.L;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int s = b / nbps;
+  const int r0 = (b - s * nbps) * 2 * lpw;
+  const float2* ws = wt + (long)s * H * rp;
+  for (int t = tid; t < lpw * H; t += nthr) {
+    int k = t / lpw;
+    int l = t - k * lpw;
+    int ra = r0 + 2 * l;
+    float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (ra < rp) q = *reinterpret_cast<const float4*>(ws + (long)k * rp + ra);
+    if (k == 0 || 2 * k == M) {
+      q.y = 0.f;
+      q.w = 0.f;
+    }
+    // z = X1 + i X2
+    sm[l * M + k] = make_float2(q.x - q.w, q.y + q.z);
+    if (k > 0 && M - k >= H) {
+      // z[M-k] = conj(X1) + i conj(X2)
+      sm[l * M + M - k] = make_float2(q.x + q.w, q.z - q.y);
+    }
+  }
+  __syncthreads();
+  fft_lds<1, NB>(sm, lpw, M, P, tid, nthr);
+  const long so = (long)s * n * n;
+  for (int t = tid; t < lpw * n; t += nthr) {
+    int l = t / n;
+    int c = t - l * n;
+    int ra = r0 + 2 * l;
+    float2 v = sm[l * M + c];
+    if (ra < n) {
+      long o = so + (long)ra * n + c;
+      out[o] = MODE == 1 ? v.x - bvec[o] : v.x;
+    }
+    if (ra + 1 < n) {
+      long o = so + (long)(ra + 1) * n + c;
+      out[o] = MODE == 1 ? v.y - bvec[o] : v.y;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+
+__global__ void reduce_partials_kernel(const double* __restrict__ p, long n,
+                                       double* out) {
+  __shared__ double red[16];
+  double acc = 0.0;
+  for (long i = threadIdx.x; i < n; i += blockDim.x) acc += p[i];
+  double tot = block_sum(acc, red);
+  if (threadIdx.x == 0) out[0] = tot;
+}
+
+void reduce_partials(const double* partials, long n, double* out,
+                     hipStream_t stream) {
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(1024), 0, stream,
+                     partials, n, out);
+  TC_HIP(hipGetLastError());
+}
+
+void Psf::apply(const float* x, const float* b, float* out, int nslices,
+                int mode, double* dot_partials, hipStream_t stream) {
+  TC_REQUIRE(nslices >= 1, "Toeplitz: need at least one slice");
+  TC_REQUIRE(mode == 0 || (mode == 1 && b != nullptr), "Toeplitz: bad mode");
+  reserve(nslices);
+  const FFTLength& F = fft_length(M_);
+  float2* wt = work_.as<float2>();
+  const int npairs = (n_ + 1) / 2;
+
+  // pass A
+  {
+    LdsLaunch ll = plan_lds_launch(F, 4, 80 * 1024);
+    int nbps = (int)cdiv(npairs, ll.lpw);
+    unsigned nblk = (unsigned)nbps * nslices;
+    ktimer_begin(KC_TOEPLITZ_ROWS_FWD, stream);
+    TC_DISPATCH_NB(ll.nb, {
+      auto k = toep_rows_fwd<NB>;
+      allow_lds((const void*)k, ll.smem);
+      hipLaunchKernelGGL(k, dim3(nblk), dim3(ll.nthr), ll.smem, stream, x, wt,
+                         F.plan(), n_, H_, rp_, ll.lpw, nbps);
+    });
+    TC_HIP(hipGetLastError());
+    ktimer_end(KC_TOEPLITZ_ROWS_FWD, stream);
+  }
+  // pass B
+  {
+    LdsLaunch ll = plan_lds_launch(F, 1, 80 * 1024);
+    int nbps = (int)cdiv(H_, ll.lpw);
+    unsigned nblk = (unsigned)nbps * nslices;
+    ktimer_begin(KC_TOEPLITZ_COLS, stream);
+    TC_DISPATCH_NB(ll.nb, {
+      auto k = toep_cols<NB>;
+      allow_lds((const void*)k, ll.smem);
+      hipLaunchKernelGGL(k, dim3(nblk), dim3(ll.nthr), ll.smem, stream, wt,
+                         spectrum(), F.plan(), n_, H_, rp_, ll.lpw, nbps,
+                         dot_partials);
+    });
+    TC_HIP(hipGetLastError());
+    ktimer_end(KC_TOEPLITZ_COLS, stream);
+  }
+  // pass C
+  {
+    LdsLaunch ll = plan_lds_launch(F, 4, 80 * 1024);
+    int nbps = (int)cdiv(npairs, ll.lpw);
+    unsigned nblk = (unsigned)nbps * nslices;
+    ktimer_begin(KC_TOEPLITZ_ROWS_INV, stream);
+    TC_DISPATCH_NB(ll.nb, {
+      if (mode == 0) {
+        auto k = toep_rows_inv<NB, 0>;
+        allow_lds((const void*)k, ll.smem);
+        hipLaunchKernelGGL(k, dim3(nblk), dim3(ll.nthr), ll.smem, stream, wt,
+                           b, out, F.plan(), n_, H_, rp_, ll.lpw, nbps);
+      } else {
+        auto k = toep_rows_inv<NB, 1>;
+        allow_lds((const void*)k, ll.smem);
+        hipLaunchKernelGGL(k, dim3(nblk), dim3(ll.nthr), ll.smem, stream, wt,
+                           b, out, F.plan(), n_, H_, rp_, ll.lpw, nbps);
+      }
+    });
+    TC_HIP(hipGetLastError());
+    ktimer_end(KC_TOEPLITZ_ROWS_INV, stream);
+  }
+}
+
+// number of partial sums written by apply() for `nslices` slices
+int toeplitz_num_partials(const Psf& p, int nslices) {
+  const FFTLength& F = fft_length(p.M());
+  LdsLaunch ll = plan_lds_launch(F, 1, 80 * 1024);
+  return (int)cdiv(p.H(), ll.lpw) * nslices;
+}
+
+}  // namespace tc

@@ -1,0 +1,314 @@
+"""Device-level operators over torch CUDA tensors (float32, C-contiguous).
+
+Thin wrappers of the C-ABI: torch supplies device memory and the current
+stream; every computation runs in libtomocam_hip.so.  Mirrors the
+single-device overloads of the reference (src/internals.h:44-92) and the
+``PointSpreadFunction`` type (src/toeplitz.h:33-160).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+def _stream() -> C.c_void_p:
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dev(t: torch.Tensor, name: str) -> C.c_void_p:
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError(f"{name} must be a CUDA tensor")
+    if t.dtype != torch.float32:
+        raise TypeError(f"{name} must be float32")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return C.c_void_p(t.data_ptr())
+
+
+def _as3d(t: torch.Tensor) -> torch.Tensor:
+    return t.unsqueeze(0) if t.dim() == 2 else t
+
+
+class PointSpreadFunction:
+    """Toeplitz normal operator of one polar geometry (src/toeplitz.h:33)."""
+
+    def __init__(self, angles=None, npix: Optional[int] = None, *,
+                 psf_array: Optional[torch.Tensor] = None):
+        self._h = C.c_void_p()
+        if psf_array is not None:
+            a = psf_array.contiguous()
+            n1 = a.shape[-1]
+            if a.dim() != 2 or a.shape[0] != n1 or n1 % 2 == 0:
+                raise ValueError("psf_array must be [2n-1, 2n-1]")
+            _lib.call("tc_psf_create_from_array", C.byref(self._h),
+                      _dev(a, "psf_array"), (n1 + 1) // 2, _stream())
+        else:
+            if angles is None or npix is None:
+                raise ValueError("need angles and npix, or psf_array")
+            ang = np.ascontiguousarray(np.asarray(angles, dtype=np.float32))
+            _lib.call("tc_psf_create", C.byref(self._h),
+                      ang.ctypes.data_as(C.c_void_p), int(ang.size),
+                      int(npix), _stream())
+        n = C.c_int()
+        m = C.c_int()
+        _lib.call("tc_psf_info", self._h, C.byref(n), C.byref(m))
+        self.n = n.value
+        self.fft_size = m.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value:
+            try:
+                _lib.fn("tc_psf_destroy")(h)
+            except Exception:
+                pass
+            self._h = C.c_void_p()
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    def array(self) -> torch.Tensor:
+        n1 = 2 * self.n - 1
+        out = torch.empty((n1, n1), dtype=torch.float32, device="cuda")
+        _lib.call("tc_psf_get_array", self._h, _dev(out, "out"), _stream())
+        return out
+
+    def _check(self, x: torch.Tensor, name: str) -> torch.Tensor:
+        x = _as3d(x)
+        if x.shape[-1] != self.n or x.shape[-2] != self.n:
+            raise ValueError(f"{name} must be [S, {self.n}, {self.n}]")
+        return x
+
+    def convolve(self, x: torch.Tensor, with_dot: bool = False):
+        """T x (PointSpreadFunction::convolve); optionally also x . T x."""
+        x = self._check(x, "x")
+        out = torch.empty_like(x)
+        dot = C.c_double()
+        _lib.call("tc_toeplitz_apply", self._h, _dev(x, "x"),
+                  _dev(out, "out"), x.shape[0],
+                  C.byref(dot) if with_dot else None, _stream())
+        return (out, dot.value) if with_dot else out
+
+    def gradient2(self, x: torch.Tensor, sinoT: torch.Tensor) -> torch.Tensor:
+        """T x - sinoT (tomocam::gradient2, src/gradient2.cpp:62)."""
+        x = self._check(x, "x")
+        sinoT = self._check(sinoT, "sinoT")
+        g = torch.empty_like(x)
+        _lib.call("tc_gradient2", self._h, _dev(x, "x"), _dev(sinoT, "sinoT"),
+                  _dev(g, "g"), x.shape[0], _stream())
+        return g
+
+    def function_value2(self, x: torch.Tensor, sinoT: torch.Tensor,
+                        sino_sq: float) -> float:
+        """||y||^2 + sum(x . T x - 2 x . sinoT) (src/error2.cpp:60-91)."""
+        x = self._check(x, "x")
+        sinoT = self._check(sinoT, "sinoT")
+        out = C.c_double()
+        _lib.call("tc_function_value2", self._h, _dev(x, "x"),
+                  _dev(sinoT, "sinoT"), x.shape[0], float(sino_sq),
+                  C.byref(out), _stream())
+        return out.value
+
+
+def add_total_var2(x: torch.Tensor, g: torch.Tensor, sigma: float,
+                   p: float = 1.2) -> torch.Tensor:
+    """g += qGGMRF gradient of x, in place (tomocam::add_total_var2)."""
+    x3 = _as3d(x)
+    g3 = _as3d(g)
+    if x3.shape != g3.shape:
+        raise ValueError("x and g must have the same shape")
+    S, R, Cc = x3.shape
+    _lib.call("tc_add_total_var2", _dev(x3, "x"), _dev(g3, "g"), S, R, Cc,
+              float(sigma), float(p), _stream())
+    return g
+
+
+def add_total_var(x: torch.Tensor, g: torch.Tensor, sigma: float,
+                  p: float = 1.2) -> torch.Tensor:
+    """tomocam::add_total_var (src/tvd_update.cpp); same arithmetic."""
+    x3 = _as3d(x)
+    g3 = _as3d(g)
+    if x3.shape != g3.shape:
+        raise ValueError("x and g must have the same shape")
+    S, R, Cc = x3.shape
+    _lib.call("tc_add_total_var", _dev(x3, "x"), _dev(g3, "g"), S, R, Cc,
+              float(sigma), float(p), _stream())
+    return g
+
+
+def tv_hessian_zero(sigma: float) -> float:
+    """Value gpu::add_tv_hessian would add (it never reaches the host)."""
+    v = C.c_float()
+    _lib.call("tc_add_tv_hessian", None, 0, float(sigma), C.byref(v))
+    return v.value
+
+
+def run2(psf: PointSpreadFunction, x0: torch.Tensor, sinoT: torch.Tensor,
+         num_iters: int, step: float, sigma: float, p: float = 1.2,
+         sino_sq: float = 0.0, refresh_every: int = 10,
+         verbose: bool = False) -> Tuple[torch.Tensor, np.ndarray,
+                                         np.ndarray, dict]:
+    """Optimizer::run2 with the mbir2 objective, device resident."""
+    x0 = psf._check(x0, "x0")
+    sinoT = psf._check(sinoT, "sinoT")
+    out = torch.empty_like(x0)
+    err = np.zeros(max(num_iters, 1), dtype=np.float64)
+    xerr = np.zeros(max(num_iters, 1), dtype=np.float64)
+    stats = np.zeros(4, dtype=np.int32)
+    _lib.call("tc_run2", psf.handle, _dev(x0, "x0"), _dev(sinoT, "sinoT"),
+              _dev(out, "out"), x0.shape[0], int(num_iters), float(step),
+              float(sigma), float(p), float(sino_sq), int(refresh_every),
+              int(bool(verbose)),
+              err.ctypes.data_as(_lib.c_double_p),
+              xerr.ctypes.data_as(_lib.c_double_p),
+              stats.ctypes.data_as(_lib.c_int_p), _stream())
+    st = dict(zip(("iterations", "attempts", "backtracks", "toeplitz"),
+                  stats.tolist()))
+    return out, err[:num_iters], xerr[:num_iters], st
+
+
+def ktimer_enable(kernel_class: int, on: bool = True) -> None:
+    _lib.call("tc_ktimer_enable", int(kernel_class), int(bool(on)))
+
+
+def ktimer_collect(kernel_class: int) -> Tuple[float, int]:
+    ms = C.c_double()
+    cnt = C.c_long()
+    _lib.call("tc_ktimer_collect", int(kernel_class), C.byref(ms),
+              C.byref(cnt))
+    return ms.value, cnt.value
+
+
+def _angles(angles) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(angles, dtype=np.float32).ravel())
+
+
+def project(img: torch.Tensor, angles) -> torch.Tensor:
+    """Radon transform (tomocam::project, src/forward.cpp:33-52):
+    [S, N, N] -> [S, P, N]."""
+    img = _as3d(img)
+    S, N, N2 = img.shape
+    if N != N2:
+        raise ValueError("images must be square")
+    ang = _angles(angles)
+    out = torch.empty((S, ang.size, N), dtype=torch.float32, device=img.device)
+    _lib.call("tc_project", _dev(img, "img"), _dev(out, "sino"), S, N,
+              ang.ctypes.data_as(C.c_void_p), int(ang.size), _stream())
+    return out
+
+
+def backproject(sino: torch.Tensor, angles, fbp: bool = False) -> torch.Tensor:
+    """Adjoint of :func:`project` (tomocam::backproject,
+    src/backward.cpp:42-67), Ram-Lak filtered if ``fbp``: [S, P, N] ->
+    [S, N, N]."""
+    sino = _as3d(sino)
+    S, P, N = sino.shape
+    ang = _angles(angles)
+    if ang.size != P:
+        raise ValueError(f"got {ang.size} angles for {P} projections")
+    out = torch.empty((S, N, N), dtype=torch.float32, device=sino.device)
+    _lib.call("tc_backproject", _dev(sino, "sino"), _dev(out, "img"), S, P, N,
+              ang.ctypes.data_as(C.c_void_p), int(bool(fbp)), _stream())
+    return out
+
+
+def gradient(x: torch.Tensor, sinoT: torch.Tensor, angles) -> torch.Tensor:
+    """Re(A^H A x)/N^3 - sinoT via two NUFFTs (tomocam::gradient)."""
+    x = _as3d(x)
+    sinoT = _as3d(sinoT)
+    S, N, _ = x.shape
+    ang = _angles(angles)
+    g = torch.empty_like(x)
+    _lib.call("tc_gradient", _dev(x, "x"), _dev(sinoT, "sinoT"), _dev(g, "g"),
+              S, N, ang.ctypes.data_as(C.c_void_p), int(ang.size), _stream())
+    return g
+
+
+def function_value(x: torch.Tensor, sino: torch.Tensor, angles) -> float:
+    """sum (P x - y)^2 (tomocam::function_value, src/error.cpp:57-59)."""
+    x = _as3d(x)
+    sino = _as3d(sino)
+    S, N, _ = x.shape
+    ang = _angles(angles)
+    out = C.c_double()
+    _lib.call("tc_function_value", _dev(x, "x"), _dev(sino, "sino"), S, N,
+              ang.ctypes.data_as(C.c_void_p), int(ang.size), C.byref(out),
+              _stream())
+    return out.value
+
+
+def xerror(a: torch.Tensor, b: torch.Tensor) -> float:
+    """sum (a - b)^2 (tomocam::xerror)."""
+    if a.shape != b.shape:
+        raise ValueError("shape mismatch")
+    out = C.c_double()
+    _lib.call("tc_xerror", _dev(a, "a"), _dev(b, "b"), a.numel(),
+              C.byref(out), _stream())
+    return out.value
+
+
+def preproc(sino: torch.Tensor, center: float) -> torch.Tensor:
+    """Zero-pad and centre the sinogram (tomocam::preproc)."""
+    sino = _as3d(sino)
+    S, P, N = sino.shape
+    npad = C.c_int()
+    _lib.call("tc_preproc", None, None, S, P, N, float(center), C.byref(npad),
+              _stream())
+    out = torch.empty((S, P, N + 2 * npad.value), dtype=torch.float32,
+                      device=sino.device)
+    _lib.call("tc_preproc", _dev(sino, "sino"), _dev(out, "out"), S, P, N,
+              float(center), C.byref(npad), _stream())
+    return out
+
+
+def postproc(img: torch.Tensor, npixels: int) -> torch.Tensor:
+    """Crop the padded reconstruction (tomocam::postproc)."""
+    img = _as3d(img)
+    S, n, _ = img.shape
+    out = torch.empty((S, npixels, npixels), dtype=torch.float32,
+                      device=img.device)
+    _lib.call("tc_postproc", _dev(img, "img"), _dev(out, "out"), S, n,
+              int(npixels), _stream())
+    return out
+
+
+def pad2d(img: torch.Tensor, padding: int) -> torch.Tensor:
+    """Symmetric zero padding by ``padding`` in total (tomocam::pad2d)."""
+    img = _as3d(img)
+    S, n, _ = img.shape
+    m = n + abs(int(padding))
+    out = torch.empty((S, m, m), dtype=torch.float32, device=img.device)
+    _lib.call("tc_pad2d", _dev(img, "img"), _dev(out, "out"), S, n,
+              int(padding), _stream())
+    return out
+
+
+def mbir(sino: torch.Tensor, angles, center: float, num_iters: int = 100,
+         sigma: float = 100.0, tol: float = 1e-4, xtol: float = 1e-4,
+         refresh_every: int = 10, verbose: bool = False):
+    """tomocam::mbir2 on a device-resident sinogram [S, P, N] -> [S, N, N].
+
+    Returns (recon, info) with info = lipschitz, step, sino_norm, max_sino,
+    padded width and accepted iterations."""
+    sino = _as3d(sino)
+    S, P, N = sino.shape
+    ang = _angles(angles)
+    if ang.size != P:
+        raise ValueError(f"got {ang.size} angles for {P} projections")
+    out = torch.empty((S, N, N), dtype=torch.float32, device=sino.device)
+    info = np.zeros(6, dtype=np.float64)
+    _lib.call("tc_mbir2", _dev(sino, "sino"), _dev(out, "recon"), S, P, N,
+              ang.ctypes.data_as(C.c_void_p), float(center), int(num_iters),
+              float(sigma), float(tol), float(xtol), int(refresh_every),
+              int(bool(verbose)), info.ctypes.data_as(_lib.c_double_p),
+              _stream())
+    keys = ("lipschitz", "step", "sino_norm", "max_sino", "padded",
+            "iterations")
+    return out, dict(zip(keys, info.tolist()))
