@@ -204,9 +204,11 @@ int tc_mbir2_host(const float* h_sino, float* h_recon, int nslices, int nproj,
  * are bracketed by HIP events on their stream. */
 int tc_ktimer_enable(int kernel_class, int on);
 
-/* Sum of the event-timed durations (ms) and number of launches of a class
+/* Sum of the event-timed durations (ms), number of launches and algorithmic
+ * HBM bytes (each input read once, each output written once) of a class
  * since the previous collect. */
-int tc_ktimer_collect(int kernel_class, double* total_ms, long* count);
+int tc_ktimer_collect(int kernel_class, double* total_ms, long* count,
+                      double* bytes);
 
 #ifdef __cplusplus
 }

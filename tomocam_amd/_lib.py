@@ -57,7 +57,7 @@ SIGNATURES = {
     "tc_mbir2_host": [vp, vp, C.c_int, C.c_int, C.c_int, vp, C.c_float,
                       C.c_int, C.c_float, C.c_float, C.c_float, C.c_int],
     "tc_ktimer_enable": [C.c_int, C.c_int],
-    "tc_ktimer_collect": [C.c_int, c_double_p, c_long_p],
+    "tc_ktimer_collect": [C.c_int, c_double_p, c_long_p, c_double_p],
 }
 
 

@@ -11,7 +11,7 @@
 
 namespace tc {
 void ktimer_enable(int kc, bool on);
-void ktimer_collect(int kc, double* total_ms, long* count);
+void ktimer_collect(int kc, double* total_ms, long* count, double* bytes);
 }  // namespace tc
 
 struct tc_psf_s {
@@ -345,13 +345,15 @@ int tc_ktimer_enable(int kernel_class, int on) {
   return guarded([&] { tc::ktimer_enable(kernel_class, on != 0); });
 }
 
-int tc_ktimer_collect(int kernel_class, double* total_ms, long* count) {
+int tc_ktimer_collect(int kernel_class, double* total_ms, long* count,
+                      double* bytes) {
   return guarded([&] {
-    double t = 0.0;
+    double t = 0.0, b = 0.0;
     long c = 0;
-    tc::ktimer_collect(kernel_class, &t, &c);
+    tc::ktimer_collect(kernel_class, &t, &c, &b);
     if (total_ms) *total_ms = t;
     if (count) *count = c;
+    if (bytes) *bytes = b;
   });
 }
 

@@ -194,7 +194,7 @@ void add_total_var(const float* x, float* g, int S, int R, int C, float sigma,
   ktimer_begin(KC_QGGMRF, stream);
   hipLaunchKernelGGL(qggmrf_kernel<0>, dim3(nblk), dim3(256), 0, stream, a);
   TC_HIP(hipGetLastError());
-  ktimer_end(KC_QGGMRF, stream);
+  ktimer_end(KC_QGGMRF, stream, 3.0 * 4.0 * S * (double)R * C);
 }
 
 float tv_hessian_zero(float sigma) {
@@ -344,7 +344,8 @@ Run2Stats run2_device(Psf& psf, const float* x0, const float* b, float* out,
       ktimer_begin(KC_MBIR_GRADIENT, stream);
       hipLaunchKernelGGL(qggmrf_kernel<1>, dim3(qblk), dim3(256), 0, stream, qa);
       TC_HIP(hipGetLastError());
-      ktimer_end(KC_MBIR_GRADIENT, stream);
+      // reads sol, T sol, b (and x, T x unless aliased), writes g
+      ktimer_end(KC_MBIR_GRADIENT, stream, (has_x ? 6.0 : 4.0) * vb);
       psf.apply(g, nullptr, Tg, S, 0, tparts, stream);
       ++st.toeplitz_applies;
       hipLaunchKernelGGL(reduce_strided_kernel, dim3(NDOT), dim3(1024), 0,
@@ -381,7 +382,8 @@ Run2Stats run2_device(Psf& psf, const float* x0, const float* b, float* out,
                          qa.xprev, Tsol, qa.Tx, g, Tg, dst, Tdst, N / 4, N, cb,
                          (float)s);
       TC_HIP(hipGetLastError());
-      ktimer_end(KC_MBIR_COMMIT, stream);
+      // reads sol, T sol, g, T g (and x, T x unless aliased), writes 2
+      ktimer_end(KC_MBIR_COMMIT, stream, (has_x ? 8.0 : 6.0) * vb);
       sol = dst;
       Tsol = Tdst;
       if (reject) {

@@ -17,6 +17,7 @@ struct KTimer {
   std::mutex mu;
   bool enabled[KC_COUNT] = {};
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs[KC_COUNT];
+  double bytes[KC_COUNT] = {};
   hipEvent_t open[KC_COUNT] = {};
 };
 
@@ -40,7 +41,7 @@ void ktimer_begin(int kc, hipStream_t stream) {
   k.open[kc] = e;
 }
 
-void ktimer_end(int kc, hipStream_t stream) {
+void ktimer_end(int kc, hipStream_t stream, double bytes) {
   KTimer& k = kt();
   if (!k.enabled[kc] || !k.open[kc]) return;
   hipEvent_t e;
@@ -48,11 +49,13 @@ void ktimer_end(int kc, hipStream_t stream) {
   TC_HIP(hipEventRecord(e, stream));
   std::lock_guard<std::mutex> lk(k.mu);
   k.pairs[kc].emplace_back(k.open[kc], e);
+  k.bytes[kc] += bytes;
   k.open[kc] = nullptr;
 }
 
-// Total milliseconds and launch count of class kc since the last collect.
-void ktimer_collect(int kc, double* total_ms, long* count) {
+// Total milliseconds, launch count and algorithmic bytes of class kc since
+// the last collect.
+void ktimer_collect(int kc, double* total_ms, long* count, double* bytes) {
   TC_REQUIRE(kc >= 0 && kc < KC_COUNT, "bad kernel class");
   KTimer& k = kt();
   std::lock_guard<std::mutex> lk(k.mu);
@@ -70,6 +73,8 @@ void ktimer_collect(int kc, double* total_ms, long* count) {
   k.pairs[kc].clear();
   *total_ms = tot;
   *count = cnt;
+  *bytes = k.bytes[kc];
+  k.bytes[kc] = 0.0;
 }
 
 }  // namespace tc

@@ -124,8 +124,10 @@ enum KernelClass : int {
 
 // Begins/ends a timed region for kernel class `kc` on `stream` if timing for
 // that class is enabled; cheap no-op otherwise.
+// `bytes` is the algorithmic HBM traffic of the launch (inputs read once,
+// outputs written once), accumulated for the roofline report.
 void ktimer_begin(int kc, hipStream_t stream);
-void ktimer_end(int kc, hipStream_t stream);
+void ktimer_end(int kc, hipStream_t stream, double bytes = 0.0);
 
 // ---------------------------------------------------------------------------
 // device complex helpers (float2 = re, im)

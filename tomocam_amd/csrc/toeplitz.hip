@@ -245,6 +245,9 @@ void Psf::apply(const float* x, const float* b, float* out, int nslices,
   const FFTLength& F = fft_length(M_);
   float2* wt = work_.as<float2>();
   const int npairs = (n_ + 1) / 2;
+  // algorithmic bytes: images n^2 floats per slice, spectrum n*H complex
+  const double img_b = 4.0 * nslices * (double)n_ * n_;
+  const double spec_b = 8.0 * nslices * (double)n_ * H_;
 
   // pass A
   {
@@ -259,7 +262,7 @@ void Psf::apply(const float* x, const float* b, float* out, int nslices,
                          F.plan(), n_, H_, rp_, ll.lpw, nbps);
     });
     TC_HIP(hipGetLastError());
-    ktimer_end(KC_TOEPLITZ_ROWS_FWD, stream);
+    ktimer_end(KC_TOEPLITZ_ROWS_FWD, stream, img_b + spec_b);
   }
   // pass B
   {
@@ -275,7 +278,8 @@ void Psf::apply(const float* x, const float* b, float* out, int nslices,
                          dot_partials);
     });
     TC_HIP(hipGetLastError());
-    ktimer_end(KC_TOEPLITZ_COLS, stream);
+    ktimer_end(KC_TOEPLITZ_COLS, stream,
+               2.0 * spec_b + 4.0 * (double)H_ * M_);
   }
   // pass C
   {
@@ -297,7 +301,8 @@ void Psf::apply(const float* x, const float* b, float* out, int nslices,
       }
     });
     TC_HIP(hipGetLastError());
-    ktimer_end(KC_TOEPLITZ_ROWS_INV, stream);
+    ktimer_end(KC_TOEPLITZ_ROWS_INV, stream,
+               spec_b + img_b * (mode == 1 ? 2.0 : 1.0));
   }
 }
 

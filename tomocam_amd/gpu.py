@@ -178,12 +178,24 @@ def ktimer_enable(kernel_class: int, on: bool = True) -> None:
     _lib.call("tc_ktimer_enable", int(kernel_class), int(bool(on)))
 
 
-def ktimer_collect(kernel_class: int) -> Tuple[float, int]:
+KERNEL_CLASSES = {
+    0: "toeplitz_rows_fwd",
+    1: "toeplitz_cols",
+    2: "toeplitz_rows_inv",
+    3: "mbir_gradient",
+    4: "mbir_commit",
+    5: "qggmrf",
+}
+
+
+def ktimer_collect(kernel_class: int) -> Tuple[float, int, float]:
+    """(total ms, launches, algorithmic bytes) since the last collect."""
     ms = C.c_double()
     cnt = C.c_long()
+    nb = C.c_double()
     _lib.call("tc_ktimer_collect", int(kernel_class), C.byref(ms),
-              C.byref(cnt))
-    return ms.value, cnt.value
+              C.byref(cnt), C.byref(nb))
+    return ms.value, cnt.value, nb.value
 
 
 def _angles(angles) -> np.ndarray:
