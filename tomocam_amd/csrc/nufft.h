@@ -65,7 +65,7 @@ class RayDft {
              long nrays, bool ramp, float scale, hipStream_t stream);
 
  private:
-  int N_, L_, sign_;
+  int N_, L_;
   DevBuf chirp_;  // float2[N]: exp(sign i pi (z-c)^2 / N)
   DevBuf bhat_;   // float2[L]: FFT_L of the conjugate chirp, / L
 };

@@ -458,7 +458,7 @@ ray_dft_kernel(const void* __restrict__ in, void* __restrict__ out,
   }
 }
 
-RayDft::RayDft(int N, int sign) : N_(N), sign_(sign) {
+RayDft::RayDft(int N, int sign) : N_(N) {
   TC_REQUIRE(N >= 1, "ray DFT: N must be >= 1");
   L_ = fft_next_smooth(2 * N - 1, false);
   const int c = N / 2;
