@@ -1,0 +1,275 @@
+// Compile-time-planned, register-resident Stockham FFT for gfx950.
+//
+// Plan: L = R_1 * R_2 * ... * R_S with every radix dividing E (elements per
+// thread); T = L / E threads cooperate on one line.  In stage s (radix R,
+// Ns = R_1 ... R_{s-1}) thread t owns butterflies j = t + i T, i < E / R:
+//   inputs   x[j + r L/R]                           (r < R)
+//   twiddle  w_L^(r (j mod Ns) L/(Ns R))
+//   outputs  y[(j - j mod Ns) R + (j mod Ns) + r Ns]
+// Each radix-R DFT (R up to 27, composite radices by Cooley-Tukey in
+// registers) runs entirely in registers; consecutive stages exchange data
+// through one LDS line (write outputs, barrier, read inputs, barrier).
+// Stage-1 inputs t + i T + r L/R_1 and last-stage outputs t + i T + r L/R_S
+// are contiguous across threads, so they are read from / written to global
+// memory directly.  Running the inverse with the radices reversed makes its
+// stage-1 input set equal to the forward's last-stage output set, so a
+// forward-multiply-inverse sequence never leaves registers in between.
+#pragma once
+
+#include "fft_device.h"
+
+namespace tc {
+namespace ct {
+
+// ----- constexpr trigonometry (double, for compile-time twiddles) -----
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+constexpr double cx_sin_small(double x) {  // |x| <= pi/4
+  double x2 = x * x, term = x, sum = x;
+  for (int k = 1; k < 12; ++k) {
+    term *= -x2 / ((2.0 * k) * (2.0 * k + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double cx_cos_small(double x) {
+  double x2 = x * x, term = 1.0, sum = 1.0;
+  for (int k = 1; k < 12; ++k) {
+    term *= -x2 / ((2.0 * k - 1.0) * (2.0 * k));
+    sum += term;
+  }
+  return sum;
+}
+// sin / cos of 2 pi num / den
+constexpr double cx_sin2pi(long num, long den) {
+  num %= den;
+  if (num < 0) num += den;
+  // octant reduction on the exact fraction
+  const double f = (double)num / (double)den;  // in [0, 1)
+  const int oct = (int)(f * 8.0);
+  const double a = (f - oct / 8.0) * 2.0 * kPi;  // [0, pi/4)
+  switch (oct) {
+    case 0: return cx_sin_small(a);
+    case 1: return cx_cos_small(kPi / 4 - a);
+    case 2: return cx_cos_small(a);
+    case 3: return cx_sin_small(kPi / 4 - a);
+    case 4: return -cx_sin_small(a);
+    case 5: return -cx_cos_small(kPi / 4 - a);
+    case 6: return -cx_cos_small(a);
+    default: return -cx_sin_small(kPi / 4 - a);
+  }
+}
+constexpr double cx_cos2pi(long num, long den) {
+  return cx_sin2pi(num * 4 + den, den * 4);
+}
+
+constexpr int smallest_factor(int n) {
+  for (int p = 2; p * p <= n; ++p)
+    if (n % p == 0) return p;
+  return n;
+}
+
+// ----- radix-R DFT in registers (natural order in and out) -----
+
+template <int R, int DIR, bool PRIME = (smallest_factor(R) == R)>
+struct DftC;
+
+// primes (and 4, 8) use the hand-written kernels
+template <int DIR>
+struct DftC<2, DIR, true> {
+  __device__ __forceinline__ static void run(float2* a) { Dft<2, DIR>::run(a); }
+};
+template <int DIR>
+struct DftC<3, DIR, true> {
+  __device__ __forceinline__ static void run(float2* a) { Dft<3, DIR>::run(a); }
+};
+template <int DIR>
+struct DftC<5, DIR, true> {
+  __device__ __forceinline__ static void run(float2* a) { Dft<5, DIR>::run(a); }
+};
+template <int DIR>
+struct DftC<7, DIR, true> {
+  __device__ __forceinline__ static void run(float2* a) { Dft<7, DIR>::run(a); }
+};
+template <int DIR>
+struct DftC<4, DIR, false> {
+  __device__ __forceinline__ static void run(float2* a) { Dft<4, DIR>::run(a); }
+};
+template <int DIR>
+struct DftC<8, DIR, false> {
+  __device__ __forceinline__ static void run(float2* a) { Dft<8, DIR>::run(a); }
+};
+
+// composite R = R1 * R2 (R1 = smallest prime factor, or 4/8 when possible):
+// n = R2 n1 + n2, k = k1 + R1 k2,
+// X[k1 + R1 k2] = sum_n2 w_R2^(n2 k2) [w_R^(n2 k1) sum_n1 a[R2 n1 + n2] w_R1^(n1 k1)]
+template <int R>
+constexpr int first_radix() {
+  return (R % 8 == 0 && R != 8) ? 8
+         : (R % 4 == 0 && R != 4) ? 4
+                                  : smallest_factor(R);
+}
+
+template <int R, int DIR>
+struct DftC<R, DIR, false> {
+  static constexpr int R1 = first_radix<R>();
+  static constexpr int R2 = R / R1;
+  __device__ __forceinline__ static void run(float2* a) {
+    float2 t[R];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      float2 b[R1];
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) b[n1] = a[R2 * n1 + n2];
+      DftC<R1, DIR>::run(b);
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) {
+        const long e = (long)n2 * k1;
+        if (e % R == 0) {
+          t[k1 * R2 + n2] = b[k1];
+        } else {
+          const float c = (float)cx_cos2pi(e, R);
+          const float s = (float)(DIR * cx_sin2pi(e, R));
+          t[k1 * R2 + n2] = make_float2(b[k1].x * c - b[k1].y * s,
+                                        b[k1].x * s + b[k1].y * c);
+        }
+      }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+      float2 c[R2];
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) c[n2] = t[k1 * R2 + n2];
+      DftC<R2, DIR>::run(c);
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) a[k1 + R1 * k2] = c[k2];
+    }
+  }
+};
+
+// ----- plan -----
+
+template <int... Rs>
+struct Radices;
+
+// true if the radices multiply to L and each divides E
+template <int L, int E, int... Rs>
+constexpr bool plan_valid() {
+  const int a[] = {Rs...};
+  long p = 1;
+  for (int i = 0; i < (int)sizeof...(Rs); ++i) {
+    if (E % a[i]) return false;
+    p *= a[i];
+  }
+  return p == L && L % E == 0;
+}
+
+// radix of stage s and product of the first s radices
+template <int... Rs>
+struct RadixList {
+  static constexpr int count = sizeof...(Rs);
+  static constexpr int at(int s) {
+    const int a[] = {Rs...};
+    return a[s];
+  }
+  static constexpr int ns(int s) {
+    const int a[] = {Rs...};
+    int p = 1;
+    for (int i = 0; i < s; ++i) p *= a[i];
+    return p;
+  }
+};
+
+// One stage's arithmetic on registers v[E] holding inputs v[i R + r] =
+// x[j + r L/R], j = t + i T; twiddle then DFT.
+template <int L, int T, int E, int R, int Ns, int DIR>
+__device__ __forceinline__ void stage_compute(float2* v, int t,
+                                              const float2* __restrict__ tw) {
+  constexpr int NB = E / R;
+  constexpr int step = L / (Ns * R);
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    if (Ns > 1) {
+      const int k = (t + i * T) % Ns;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        const float2 w = tw[k * r * step];
+        v[i * R + r] = DIR < 0 ? cmul(v[i * R + r], w) : cmulc(v[i * R + r], w);
+      }
+    }
+    DftC<R, DIR>::run(v + i * R);
+  }
+}
+
+// Writes stage outputs (radix R, Ns) to LDS at their Stockham destinations.
+template <int L, int T, int E, int R, int Ns>
+__device__ __forceinline__ void stage_write(const float2* v, float2* line,
+                                            int t) {
+  constexpr int NB = E / R;
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const int j = t + i * T;
+    const int k = j % Ns;
+    const int base = (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) line[base + r * Ns] = v[i * R + r];
+  }
+}
+
+// Reads the inputs of a radix-R stage from LDS.
+template <int L, int T, int E, int R>
+__device__ __forceinline__ void stage_read(float2* v, const float2* line,
+                                           int t) {
+  constexpr int NB = E / R;
+#pragma unroll
+  for (int i = 0; i < NB; ++i)
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[i * R + r] = line[t + i * T + r * (L / R)];
+}
+
+// Runs stages s..S-1 of the plan (radices Rs in order): stage s computes on
+// registers; later stages first exchange through `line`.  Every thread of
+// the workgroup must call this (it contains barriers); `active` masks
+// threads beyond the lines of a partial workgroup.
+template <int L, int T, int E, int DIR, int s, int... Rs>
+struct StagesDispatch {
+  using RL = RadixList<Rs...>;
+  __device__ __forceinline__ static void run(float2* v, float2* line, int t,
+                                             bool active,
+                                             const float2* __restrict__ tw) {
+    if constexpr (s < RL::count) {
+      constexpr int R = RL::at(s);
+      constexpr int Ns = RL::ns(s);
+      if constexpr (s > 0) {
+        constexpr int Rp = RL::at(s - 1);
+        constexpr int Nsp = RL::ns(s - 1);
+        if (active) stage_write<L, T, E, Rp, Nsp>(v, line, t);
+        __syncthreads();
+        if (active) stage_read<L, T, E, R>(v, line, t);
+        __syncthreads();
+      }
+      if (active) stage_compute<L, T, E, R, Ns, DIR>(v, t, tw);
+      StagesDispatch<L, T, E, DIR, s + 1, Rs...>::run(v, line, t, active, tw);
+    }
+  }
+};
+
+// Full transform on registers.  Before: v[i R1 + r] = x[t + i T + r L/R1].
+// After: v[i RS + r] = X[t + i T + r L/RS].
+template <int L, int E, int DIR, int... Rs>
+__device__ __forceinline__ void transform(float2* v, float2* line, int t,
+                                          bool active,
+                                          const float2* __restrict__ tw) {
+  static_assert(plan_valid<L, E, Rs...>(), "invalid FFT plan");
+  StagesDispatch<L, L / E, E, DIR, 0, Rs...>::run(v, line, t, active, tw);
+}
+
+// index of register element (i, r) for a stage of radix R
+template <int L, int T, int R>
+__device__ __forceinline__ int elem_index(int t, int i, int r) {
+  return t + i * T + r * (L / R);
+}
+
+}  // namespace ct
+}  // namespace tc

@@ -237,17 +237,26 @@ void reduce_partials(const double* partials, long n, double* out,
   TC_HIP(hipGetLastError());
 }
 
+int toeplitz_ct_lpw(int M);
+bool toeplitz_apply_ct(int M, const float* x, const float* b, float* out,
+                       float2* wt, const float* spec, int n, int H, int rp,
+                       int nslices, int mode, double* partials,
+                       hipStream_t stream, double img_b, double spec_b);
+
 void Psf::apply(const float* x, const float* b, float* out, int nslices,
                 int mode, double* dot_partials, hipStream_t stream) {
   TC_REQUIRE(nslices >= 1, "Toeplitz: need at least one slice");
   TC_REQUIRE(mode == 0 || (mode == 1 && b != nullptr), "Toeplitz: bad mode");
   reserve(nslices);
-  const FFTLength& F = fft_length(M_);
   float2* wt = work_.as<float2>();
-  const int npairs = (n_ + 1) / 2;
   // algorithmic bytes: images n^2 floats per slice, spectrum n*H complex
   const double img_b = 4.0 * nslices * (double)n_ * n_;
   const double spec_b = 8.0 * nslices * (double)n_ * H_;
+  if (toeplitz_apply_ct(M_, x, b, out, wt, spectrum(), n_, H_, rp_, nslices,
+                        mode, dot_partials, stream, img_b, spec_b))
+    return;
+  const FFTLength& F = fft_length(M_);
+  const int npairs = (n_ + 1) / 2;
 
   // pass A
   {
@@ -308,6 +317,8 @@ void Psf::apply(const float* x, const float* b, float* out, int nslices,
 
 // number of partial sums written by apply() for `nslices` slices
 int toeplitz_num_partials(const Psf& p, int nslices) {
+  const int ct = toeplitz_ct_lpw(p.M());
+  if (ct > 0) return (int)cdiv(p.H(), ct) * nslices;
   const FFTLength& F = fft_length(p.M());
   LdsLaunch ll = plan_lds_launch(F, 1, 80 * 1024);
   return (int)cdiv(p.H(), ll.lpw) * nslices;

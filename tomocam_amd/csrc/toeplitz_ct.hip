@@ -1,0 +1,311 @@
+// Toeplitz passes on compile-time FFT plans (fft_ct.h) for the workload
+// sizes; other sizes use the generic kernels in toeplitz.hip.
+#include "fft_ct.h"
+#include "lds_launch.h"
+#include "toeplitz.h"
+
+namespace tc {
+
+using ct::Radices;
+
+template <int L, int E, int DIR, class Rad>
+struct Xform;
+template <int L, int E, int DIR, int... Rs>
+struct Xform<L, E, DIR, Radices<Rs...>> {
+  static constexpr int first = ct::RadixList<Rs...>::at(0);
+  static constexpr int last = ct::RadixList<Rs...>::at(sizeof...(Rs) - 1);
+  __device__ __forceinline__ static void run(float2* v, float2* line, int t,
+                                             bool active,
+                                             const float2* __restrict__ tw) {
+    ct::transform<L, E, DIR, Rs...>(v, line, t, active, tw);
+  }
+};
+
+// Plans: forward radices Fwd, inverse radices Inv = reversed Fwd, LPW lines
+// per workgroup.
+template <int L_, int E_, int LPW_, class Fwd_, class Inv_>
+struct TPlan {
+  static constexpr int L = L_, E = E_, LPW = LPW_, T = L_ / E_;
+  static constexpr int NT = LPW_ * (L_ / E_);
+  using Fwd = Fwd_;
+  using Inv = Inv_;
+};
+
+using Plan750 = TPlan<750, 30, 10, Radices<10, 15, 5>, Radices<5, 15, 10>>;
+using Plan1458 = TPlan<1458, 18, 3, Radices<18, 9, 9>, Radices<9, 9, 18>>;
+using Plan2916 = TPlan<2916, 18, 3, Radices<18, 18, 9>, Radices<9, 18, 18>>;
+using Plan5832 = TPlan<5832, 18, 1, Radices<18, 18, 18>, Radices<18, 18, 18>>;
+
+// ---------------------------------------------------------------------------
+// pass A: pair-packed forward row FFTs -> Wt[s][k1][row]
+// ---------------------------------------------------------------------------
+
+template <class P>
+__global__ void __launch_bounds__(P::NT)
+ct_rows_fwd(const float* __restrict__ x, float2* __restrict__ wt,
+            const float2* __restrict__ tw, int n, int H, int rp, int nbps) {
+  extern __shared__ float2 sm[];
+  constexpr int L = P::L, E = P::E, T = P::T, LPW = P::LPW;
+  using F = Xform<L, E, -1, typename P::Fwd>;
+  constexpr int R1 = F::first, RS = F::last;
+  const int line = threadIdx.x / T;
+  const int t = threadIdx.x - line * T;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int s = b / nbps;
+  const int r0 = (b - s * nbps) * 2 * LPW;
+  const int ra = r0 + 2 * line;
+  const float* xs = x + (long)s * n * n;
+  const float* rowa = xs + (long)ra * n;
+  const bool hasa = ra < n, hasb = ra + 1 < n;
+  float2 v[E];
+#pragma unroll
+  for (int i = 0; i < E / R1; ++i)
+#pragma unroll
+    for (int r = 0; r < R1; ++r) {
+      const int c = t + i * T + r * (L / R1);
+      float re = 0.f, im = 0.f;
+      if (c < n) {
+        if (hasa) re = rowa[c];
+        if (hasb) im = rowa[n + c];
+      }
+      v[i * R1 + r] = make_float2(re, im);
+    }
+  float2* ln = sm + line * L;
+  F::run(v, ln, t, true, tw);
+#pragma unroll
+  for (int i = 0; i < E / RS; ++i)
+#pragma unroll
+    for (int r = 0; r < RS; ++r) ln[t + i * T + r * (L / RS)] = v[i * RS + r];
+  __syncthreads();
+  float2* ws = wt + (long)s * H * rp;
+  for (int q = threadIdx.x; q < LPW * H; q += P::NT) {
+    const int k = q / LPW;
+    const int l = q - k * LPW;
+    const int rr = r0 + 2 * l;
+    if (rr < rp) {
+      const float2 zk = sm[l * L + k];
+      const float2 zm = cconj(sm[l * L + (k == 0 ? 0 : L - k)]);
+      const float2 sp = cadd(zk, zm), df = csub(zk, zm);
+      *reinterpret_cast<float4*>(ws + (long)k * rp + rr) =
+          make_float4(0.5f * sp.x, 0.5f * sp.y, 0.5f * df.y, -0.5f * df.x);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pass B: column FFT, spectrum multiply (+ spectral x.Tx), inverse FFT, all
+// without leaving registers between the two transforms
+// ---------------------------------------------------------------------------
+
+template <class P>
+__global__ void __launch_bounds__(P::NT)
+ct_cols(float2* __restrict__ wt, const float* __restrict__ spec,
+        const float2* __restrict__ tw, int n, int H, int rp, int nbps,
+        double* partials) {
+  extern __shared__ float2 sm[];
+  __shared__ double red[16];
+  constexpr int L = P::L, E = P::E, T = P::T, LPW = P::LPW;
+  using F = Xform<L, E, -1, typename P::Fwd>;
+  using I = Xform<L, E, 1, typename P::Inv>;
+  constexpr int R1 = F::first, RS = F::last;
+  static_assert(I::first == RS && I::last == R1, "inverse must reverse radices");
+  const int line = threadIdx.x / T;
+  const int t = threadIdx.x - line * T;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int s = b / nbps;
+  const int k1 = (b - s * nbps) * LPW + line;
+  const bool active = k1 < H;
+  float2* col = wt + ((long)s * H + (active ? k1 : 0)) * rp;
+  float2 v[E];
+#pragma unroll
+  for (int i = 0; i < E / R1; ++i)
+#pragma unroll
+    for (int r = 0; r < R1; ++r) {
+      const int idx = t + i * T + r * (L / R1);
+      v[i * R1 + r] = (active && idx < n) ? col[idx] : make_float2(0.f, 0.f);
+    }
+  float2* ln = sm + line * L;
+  F::run(v, ln, t, active, tw);
+  double acc = 0.0;
+  if (active) {
+    const float* sp = spec + (long)k1 * L;
+    const float wgt = (k1 == 0 || 2 * k1 == L) ? 1.f : 2.f;
+    float facc = 0.f;
+#pragma unroll
+    for (int i = 0; i < E / RS; ++i)
+#pragma unroll
+      for (int r = 0; r < RS; ++r) {
+        const int k2 = t + i * T + r * (L / RS);
+        const float h = sp[k2];
+        const float2 q = v[i * RS + r];
+        facc += h * (q.x * q.x + q.y * q.y);
+        v[i * RS + r] = cscale(q, h);
+      }
+    acc = (double)(wgt * facc);
+  }
+  // the inverse's first stage needs no exchange (same element set)
+  I::run(v, ln, t, active, tw);
+  if (active) {
+#pragma unroll
+    for (int i = 0; i < E / R1; ++i)
+#pragma unroll
+      for (int r = 0; r < R1; ++r) {
+        const int idx = t + i * T + r * (L / R1);
+        if (idx < n) col[idx] = v[i * R1 + r];
+      }
+  }
+  if (partials) {
+    double tot = block_sum(acc, red);
+    if (threadIdx.x == 0) partials[b] = tot;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pass C: Hermitian completion, pair-packed inverse row FFTs, crop, epilogue
+// ---------------------------------------------------------------------------
+
+template <class P, int MODE>
+__global__ void __launch_bounds__(P::NT)
+ct_rows_inv(const float2* __restrict__ wt, const float* __restrict__ bvec,
+            float* __restrict__ out, const float2* __restrict__ tw, int n,
+            int H, int rp, int nbps) {
+  extern __shared__ float2 sm[];
+  constexpr int L = P::L, E = P::E, T = P::T, LPW = P::LPW;
+  using I = Xform<L, E, 1, typename P::Inv>;
+  constexpr int R1 = I::first, RS = I::last;
+  const int line = threadIdx.x / T;
+  const int t = threadIdx.x - line * T;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int s = b / nbps;
+  const int r0 = (b - s * nbps) * 2 * LPW;
+  const float2* ws = wt + (long)s * H * rp;
+  for (int q = threadIdx.x; q < LPW * H; q += P::NT) {
+    const int k = q / LPW;
+    const int l = q - k * LPW;
+    const int rr = r0 + 2 * l;
+    float4 w4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (rr < rp) w4 = *reinterpret_cast<const float4*>(ws + (long)k * rp + rr);
+    if (k == 0 || 2 * k == L) {
+      w4.y = 0.f;
+      w4.w = 0.f;
+    }
+    sm[l * L + k] = make_float2(w4.x - w4.w, w4.y + w4.z);
+    if (k > 0 && L - k >= H) sm[l * L + L - k] = make_float2(w4.x + w4.w, w4.z - w4.y);
+  }
+  __syncthreads();
+  float2* ln = sm + line * L;
+  float2 v[E];
+#pragma unroll
+  for (int i = 0; i < E / R1; ++i)
+#pragma unroll
+    for (int r = 0; r < R1; ++r) v[i * R1 + r] = ln[t + i * T + r * (L / R1)];
+  __syncthreads();
+  I::run(v, ln, t, true, tw);
+  const int ra = r0 + 2 * line;
+  const long so = (long)s * n * n;
+#pragma unroll
+  for (int i = 0; i < E / RS; ++i)
+#pragma unroll
+    for (int r = 0; r < RS; ++r) {
+      const int c = t + i * T + r * (L / RS);
+      if (c < n) {
+        const float2 q = v[i * RS + r];
+        if (ra < n) {
+          const long o = so + (long)ra * n + c;
+          out[o] = MODE == 1 ? q.x - bvec[o] : q.x;
+        }
+        if (ra + 1 < n) {
+          const long o = so + (long)(ra + 1) * n + c;
+          out[o] = MODE == 1 ? q.y - bvec[o] : q.y;
+        }
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
+
+template <class P>
+static void run_ct(const float* x, const float* b, float* out, float2* wt,
+                   const float* spec, int n, int H, int rp, int nslices,
+                   int mode, double* partials, hipStream_t stream,
+                   double img_b, double spec_b) {
+  const float2* tw = fft_length(P::L).plan().tw;
+  const size_t smem = sizeof(float2) * P::LPW * P::L;
+  const int npairs = (n + 1) / 2;
+  {
+    const int nbps = (int)cdiv(npairs, P::LPW);
+    auto k = ct_rows_fwd<P>;
+    allow_lds((const void*)k, smem);
+    ktimer_begin(KC_TOEPLITZ_ROWS_FWD, stream);
+    hipLaunchKernelGGL(k, dim3(nbps * nslices), dim3(P::NT), smem, stream, x,
+                       wt, tw, n, H, rp, nbps);
+    TC_HIP(hipGetLastError());
+    ktimer_end(KC_TOEPLITZ_ROWS_FWD, stream, img_b + spec_b);
+  }
+  {
+    const int nbps = (int)cdiv(H, P::LPW);
+    auto k = ct_cols<P>;
+    allow_lds((const void*)k, smem);
+    ktimer_begin(KC_TOEPLITZ_COLS, stream);
+    hipLaunchKernelGGL(k, dim3(nbps * nslices), dim3(P::NT), smem, stream, wt,
+                       spec, tw, n, H, rp, nbps, partials);
+    TC_HIP(hipGetLastError());
+    ktimer_end(KC_TOEPLITZ_COLS, stream, 2.0 * spec_b + 4.0 * (double)H * P::L);
+  }
+  {
+    const int nbps = (int)cdiv(npairs, P::LPW);
+    ktimer_begin(KC_TOEPLITZ_ROWS_INV, stream);
+    if (mode == 0) {
+      auto k = ct_rows_inv<P, 0>;
+      allow_lds((const void*)k, smem);
+      hipLaunchKernelGGL(k, dim3(nbps * nslices), dim3(P::NT), smem, stream,
+                         wt, b, out, tw, n, H, rp, nbps);
+    } else {
+      auto k = ct_rows_inv<P, 1>;
+      allow_lds((const void*)k, smem);
+      hipLaunchKernelGGL(k, dim3(nbps * nslices), dim3(P::NT), smem, stream,
+                         wt, b, out, tw, n, H, rp, nbps);
+    }
+    TC_HIP(hipGetLastError());
+    ktimer_end(KC_TOEPLITZ_ROWS_INV, stream,
+               spec_b + img_b * (mode == 1 ? 2.0 : 1.0));
+  }
+}
+
+int toeplitz_ct_lpw(int M) {
+  switch (M) {
+    case 750: return Plan750::LPW;
+    case 1458: return Plan1458::LPW;
+    case 2916: return Plan2916::LPW;
+    case 5832: return Plan5832::LPW;
+    default: return 0;
+  }
+}
+
+bool toeplitz_apply_ct(int M, const float* x, const float* b, float* out,
+                       float2* wt, const float* spec, int n, int H, int rp,
+                       int nslices, int mode, double* partials,
+                       hipStream_t stream, double img_b, double spec_b) {
+  switch (M) {
+    case 750:
+      run_ct<Plan750>(x, b, out, wt, spec, n, H, rp, nslices, mode, partials,
+                      stream, img_b, spec_b);
+      return true;
+    case 1458:
+      run_ct<Plan1458>(x, b, out, wt, spec, n, H, rp, nslices, mode, partials,
+                       stream, img_b, spec_b);
+      return true;
+    case 2916:
+      run_ct<Plan2916>(x, b, out, wt, spec, n, H, rp, nslices, mode, partials,
+                       stream, img_b, spec_b);
+      return true;
+    case 5832:
+      run_ct<Plan5832>(x, b, out, wt, spec, n, H, rp, nslices, mode, partials,
+                       stream, img_b, spec_b);
+      return true;
+    default:
+      return false;
+  }
+}
+
+}  // namespace tc
