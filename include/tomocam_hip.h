@@ -175,17 +175,19 @@ int tc_run2(tc_psf p, const float* d_x0, const float* d_sinoT, float* d_out,
             double sino_sq, int refresh_every, int verbose, double* h_err,
             double* h_xerr, int* h_stats, void* stream);
 
-/* tomocam::mbir2 (src/mbir2.cpp:40-145) without an initial guess, as called
- * by cTomocam.mbir (src/pyTomocam.cpp:116-134): normalise by max(sino), FBP
- * initial guess, pad/centre, P^T y, PSF, step = min(1, 1/L), run2, crop.
- * d_sino[S][P][N] -> d_recon[S][N][N], both on the device.  tol/xtol are
- * ignored like the reference; an all-zero sinogram returns TC_ERR_VALUE.
- * Optional h_info[6] = {lipschitz, step, ||y||^2, max(sino), padded width,
- * accepted iterations}.  Synchronises `stream`. */
-int tc_mbir2(const float* d_sino, float* d_recon, int nslices, int nproj,
-             int nrays, const float* h_angles, float center, int num_iters,
-             float sigma, float tol, float xtol, int refresh_every,
-             int verbose, double* h_info, void* stream);
+/* tomocam::mbir2 (src/mbir2.cpp:40-145), as called by cTomocam.mbir
+ * (src/pyTomocam.cpp:116-134): normalise by max(sino), initial guess (the
+ * FBP of the normalised sinogram, or d_x0[S][N][N] if not NULL), pad/centre,
+ * P^T y, PSF, step = min(1, 1/L), run2, crop.  d_sino[S][P][N] ->
+ * d_recon[S][N][N], all on the device.  tol/xtol are ignored like the
+ * reference.  An all-zero sinogram returns TC_ERR_VALUE without a guess and
+ * copies the guess to d_recon with one (src/mbir2.cpp:50-53).  Optional
+ * h_info[6] = {lipschitz, step, ||y||^2, max(sino), padded width, accepted
+ * iterations}.  Synchronises `stream`. */
+int tc_mbir2(const float* d_sino, const float* d_x0, float* d_recon,
+             int nslices, int nproj, int nrays, const float* h_angles,
+             float center, int num_iters, float sigma, float tol, float xtol,
+             int refresh_every, int verbose, double* h_info, void* stream);
 
 /* Same with host arrays (copies in and out, like the reference's DArray
  * interface). */

@@ -51,8 +51,8 @@ SIGNATURES = {
     "tc_preproc": [vp, vp, C.c_int, C.c_int, C.c_int, C.c_float, c_int_p, vp],
     "tc_postproc": [vp, vp, C.c_int, C.c_int, C.c_int, vp],
     "tc_pad2d": [vp, vp, C.c_int, C.c_int, C.c_int, vp],
-    "tc_mbir2": [vp, vp, C.c_int, C.c_int, C.c_int, vp, C.c_float, C.c_int,
-                 C.c_float, C.c_float, C.c_float, C.c_int, C.c_int,
+    "tc_mbir2": [vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, C.c_float,
+                 C.c_int, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int,
                  c_double_p, vp],
     "tc_mbir2_host": [vp, vp, C.c_int, C.c_int, C.c_int, vp, C.c_float,
                       C.c_int, C.c_float, C.c_float, C.c_float, C.c_int],

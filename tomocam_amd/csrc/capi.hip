@@ -296,10 +296,10 @@ static void fill_info(const tc::Mbir2Result& r, double* h_info) {
   h_info[5] = r.stats.iterations;
 }
 
-int tc_mbir2(const float* d_sino, float* d_recon, int nslices, int nproj,
-             int nrays, const float* h_angles, float center, int num_iters,
-             float sigma, float tol, float xtol, int refresh_every,
-             int verbose, double* h_info, void* stream) {
+int tc_mbir2(const float* d_sino, const float* d_x0, float* d_recon,
+             int nslices, int nproj, int nrays, const float* h_angles,
+             float center, int num_iters, float sigma, float tol, float xtol,
+             int refresh_every, int verbose, double* h_info, void* stream) {
   return guarded([&] {
     (void)tol;
     (void)xtol;
@@ -310,8 +310,9 @@ int tc_mbir2(const float* d_sino, float* d_recon, int nslices, int nproj,
     o.sigma = sigma;
     o.refresh_every = refresh_every;
     o.verbose = verbose;
-    tc::Mbir2Result r = tc::mbir2(d_sino, d_recon, nslices, nproj, nrays,
-                                  h_angles, center, o, as_stream(stream));
+    tc::Mbir2Result r = tc::mbir2(d_sino, d_x0, d_recon, nslices, nproj,
+                                  nrays, h_angles, center, o,
+                                  as_stream(stream));
     fill_info(r, h_info);
   });
 }
@@ -334,8 +335,8 @@ int tc_mbir2_host(const float* h_sino, float* h_recon, int nslices, int nproj,
     o.num_iters = num_iters;
     o.sigma = sigma;
     o.verbose = verbose;
-    tc::mbir2(ds.as<float>(), dr.as<float>(), nslices, nproj, nrays, h_angles,
-              center, o, nullptr);
+    tc::mbir2(ds.as<float>(), nullptr, dr.as<float>(), nslices, nproj, nrays,
+              h_angles, center, o, nullptr);
     TC_HIP(hipMemcpy(h_recon, dr.as<void>(), sizeof(float) * nr,
                      hipMemcpyDeviceToHost));
   });

@@ -40,10 +40,11 @@ struct Mbir2Result {
   int padded = 0;
 };
 
-// tomocam::mbir2 (src/mbir2.cpp:40-145) without an initial guess:
-// sino [S][P][N] (device) -> recon [S][N][N] (device).
-Mbir2Result mbir2(const float* sino, float* recon, int S, int P, int N,
-                  const float* h_angles, float center, const Mbir2Options& o,
-                  hipStream_t stream);
+// tomocam::mbir2 (src/mbir2.cpp:40-145): sino [S][P][N] (device) -> recon
+// [S][N][N] (device); x0 [S][N][N] is the initial guess, or NULL for the
+// FBP of the normalised sinogram.
+Mbir2Result mbir2(const float* sino, const float* x0, float* recon, int S,
+                  int P, int N, const float* h_angles, float center,
+                  const Mbir2Options& o, hipStream_t stream);
 
 }  // namespace tc

@@ -88,9 +88,9 @@ __global__ void fill_kernel(float* a, long n, float v) {
   if (i < n) a[i] = v;
 }
 
-Mbir2Result mbir2(const float* sino, float* recon, int S, int P, int N,
-                  const float* h_angles, float center, const Mbir2Options& o,
-                  hipStream_t stream) {
+Mbir2Result mbir2(const float* sino, const float* x0, float* recon, int S,
+                  int P, int N, const float* h_angles, float center,
+                  const Mbir2Options& o, hipStream_t stream) {
   TC_REQUIRE(S >= 1 && P >= 1 && N >= 2, "mbir2: bad sinogram shape");
   TC_REQUIRE(h_angles != nullptr, "mbir2: null angles");
   Mbir2Result res;
@@ -100,15 +100,27 @@ Mbir2Result mbir2(const float* sino, float* recon, int S, int P, int N,
   float mx = 0.f, mn = 0.f;
   minmax(sino, nsino, &mx, &mn, stream);
   res.max_sino = mx;
-  if (mx == 0.f)
-    throw Error(TC_ERR_VALUE, "mbir2: sinogram has ZERO max value");
+  const size_t img_bytes = sizeof(float) * (size_t)S * N * N;
+  if (mx == 0.f) {
+    // src/mbir2.cpp:50-53: return the guess (none: error)
+    if (!x0) throw Error(TC_ERR_VALUE, "mbir2: sinogram has ZERO max value");
+    TC_HIP(hipMemcpyAsync(recon, x0, img_bytes, hipMemcpyDeviceToDevice,
+                          stream));
+    TC_HIP(hipStreamSynchronize(stream));
+    return res;
+  }
   DevBuf bsino2(sizeof(float) * nsino);
   affine(sino, bsino2.as<float>(), nsino, 0.f, 1.0f / mx, stream);
 
-  // FBP initial guess at the unpadded width (src/mbir2.cpp:60)
-  DevBuf bx0(sizeof(float) * (size_t)S * N * N);
-  backproject(bsino2.as<float>(), bx0.as<float>(), S, P, N, h_angles, true,
-              stream);
+  // initial guess: given, or the FBP at the unpadded width
+  // (src/mbir2.cpp:56-60)
+  DevBuf bx0(img_bytes);
+  if (x0)
+    TC_HIP(hipMemcpyAsync(bx0.as<void>(), x0, img_bytes,
+                          hipMemcpyDeviceToDevice, stream));
+  else
+    backproject(bsino2.as<float>(), bx0.as<float>(), S, P, N, h_angles, true,
+                stream);
 
   // pad and centre (src/mbir2.cpp:63-66)
   int npad = 0, off = 0;

@@ -49,6 +49,14 @@ class PolarNufft {
   DevBuf fine_;      // float2 [batch][nf][nf]
   int fine_batch_ = 0;
   float2* fine(int batch);
+  // Spreading work list, built once per geometry: one item per (fine-grid
+  // tile, chunk of at most SPREAD_CHUNK points) so that the tiles near the
+  // centre, which every ray crosses, are split over many workgroups.
+  bool work_ready_ = false;
+  int nwork_ = 0, nslots_ = 0, nfix_ = 0;
+  DevBuf work_;  // int4: tile, first point, end point, scratch slot (-1: none)
+  DevBuf fix_;   // int4: tile, first slot, slot count, 0
+  void build_work(hipStream_t stream);
 };
 
 // Centred DFT along rays by Bluestein's algorithm:

@@ -98,90 +98,269 @@ __device__ __forceinline__ void ray_interval(double a, double U0, double lo,
   thi = fmin(thi, t2);
 }
 
-__global__ void __launch_bounds__(256) spread_kernel(SpreadArgs a) {
-  __shared__ float2 acc[SP_SB][SP_T * SP_T];
-  __shared__ int2 seg[SP_LIST];
-  __shared__ int nseg;
+// Inclusive prefix sum of one int per thread over a 256-thread block.
+__device__ __forceinline__ int block_scan_incl(int v, int* wsum) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int u = __shfl_up(v, d, 64);
+    if (lane >= d) v += u;
+  }
+  if (lane == 63) wsum[wid] = v;
+  __syncthreads();
+  int off = 0;
+  for (int w = 0; w < wid; ++w) off += wsum[w];
+  __syncthreads();
+  return v + off;
+}
+
+constexpr int SPREAD_CHUNK = 1024;  // points per work item
+
+// Enumerates the ray segments (superset j ranges) of shift (U0, V0) whose
+// kernel footprints meet the tile window; leaves seg_pj[k] = p | jlo << 11
+// and exclusive offsets seg_off[0..nseg] in LDS and returns the number of
+// points.  Called by all 256 threads (contains barriers).
+__device__ int enumerate_segments(const SpreadArgs& a, double U0, double V0,
+                                  double lox, double hix, double loy,
+                                  double hiy, int* seg_pj, int* seg_off,
+                                  int* wsum, int* nseg_sh, int* nsg_out) {
   const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wid = tid >> 6;
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
-  const int s0 = blockIdx.y * SP_SB;
-  const int ns = min(SP_SB, a.nsl - s0);
+  // Deterministic compaction: thread t examines angles t, t+256, ... and
+  // writes its non-empty segments at a block-scanned offset, so every
+  // workgroup of the same tile sees the same segment order.
+  constexpr int MAXA = SP_LIST / 256;
+  int jl[MAXA], jh[MAXA];
+  int local = 0;
+#pragma unroll
+  for (int i = 0; i < MAXA; ++i) {
+    const int p = tid + 256 * i;
+    jl[i] = 1;
+    jh[i] = 0;
+    if (p < a.P) {
+      double tlo = -1e30, thi = 1e30;
+      ray_interval(a.ax[p], U0, lox, hix, tlo, thi);
+      ray_interval(a.ay[p], V0, loy, hiy, tlo, thi);
+      if (tlo <= thi) {
+        tlo = fmax(tlo, -(double)a.N - 2.0);
+        thi = fmin(thi, (double)a.N + 2.0);
+        jl[i] = max(0, (int)floor(tlo) + a.cidx - 1);
+        jh[i] = min(a.N - 1, (int)ceil(thi) + a.cidx + 1);
+      }
+    }
+    local += jl[i] <= jh[i];
+  }
+  __syncthreads();  // previous users of the segment arrays are done
+  const int incl0 = block_scan_incl(local, wsum);
+  int pos = incl0 - local;
+#pragma unroll
+  for (int i = 0; i < MAXA; ++i) {
+    if (jl[i] <= jh[i]) {
+      seg_pj[pos] = (tid + 256 * i) | (jl[i] << 11);
+      seg_off[pos] = jh[i] - jl[i] + 1;
+      ++pos;
+    }
+  }
+  if (tid == 255) *nseg_sh = incl0;
+  __syncthreads();
+  const int nsg = *nseg_sh;
+  const int per = (nsg + 255) >> 8;
+  const int start = tid * per;
+  int lsum = 0;
+  for (int k = 0; k < per; ++k)
+    if (start + k < nsg) lsum += seg_off[start + k];
+  const int incl = block_scan_incl(lsum, wsum);
+  int run = incl - lsum;
+  for (int k = 0; k < per; ++k) {
+    const int idx = start + k;
+    if (idx < nsg) {
+      const int c = seg_off[idx];
+      seg_off[idx] = run;
+      run += c;
+    }
+  }
+  if (tid == 255) seg_off[nsg] = incl;
+  __syncthreads();
+  *nsg_out = nsg;
+  return seg_off[nsg];
+}
+
+struct TileWindow {
+  int X0, Y0, X1, Y1;
+  double lox, hix, loy, hiy;
+};
+
+__device__ __forceinline__ TileWindow tile_window(const SpreadArgs& a,
+                                                  int tile) {
+  TileWindow w;
   const int ty = tile / a.ntx;
   const int tx = tile - ty * a.ntx;
-  const int X0 = tx * SP_T, Y0 = ty * SP_T;
-  const int X1 = min(X0 + SP_T, a.nf), Y1 = min(Y0 + SP_T, a.nf);
+  w.X0 = tx * SP_T;
+  w.Y0 = ty * SP_T;
+  w.X1 = min(w.X0 + SP_T, a.nf);
+  w.Y1 = min(w.Y0 + SP_T, a.nf);
+  const double hw = 0.5 * NUFFT_W;
+  w.lox = w.X0 - hw;
+  w.hix = w.X1 - 1 + hw;
+  w.loy = w.Y0 - hw;
+  w.hiy = w.Y1 - 1 + hw;
+  return w;
+}
+
+__device__ __forceinline__ bool shift_skipped(const SpreadArgs& a,
+                                              const TileWindow& w, double U0,
+                                              double V0) {
+  return U0 + a.umax <= w.lox || U0 - a.umax > w.hix ||
+         V0 + a.umax <= w.loy || V0 - a.umax > w.hiy;
+}
+
+// points per tile (all shifts), for building the work list
+__global__ void __launch_bounds__(256) spread_count_kernel(SpreadArgs a,
+                                                           int* counts) {
+  __shared__ int seg_pj[SP_LIST];
+  __shared__ int seg_off[SP_LIST + 1];
+  __shared__ int wsum[4];
+  __shared__ int nseg;
+  const int tile = blockIdx.x;
+  const TileWindow w = tile_window(a, tile);
+  int tot = 0;
+  for (int ky = -1; ky <= 1; ++ky)
+    for (int kx = -1; kx <= 1; ++kx) {
+      const double U0 = 0.5 * a.nf + kx * (double)a.nf;
+      const double V0 = 0.5 * a.nf + ky * (double)a.nf;
+      if (shift_skipped(a, w, U0, V0)) continue;
+      int nsg;
+      tot += enumerate_segments(a, U0, V0, w.lox, w.hix, w.loy, w.hiy, seg_pj,
+                                seg_off, wsum, &nseg, &nsg);
+    }
+  if (threadIdx.x == 0) counts[tile] = tot;
+}
+
+// One workgroup per (work item, slice group): accumulates the points
+// [f0, f1) of its tile in LDS (one point per lane) and writes the tile to the
+// fine grid (first chunk) or to a scratch slot (further chunks).
+__global__ void __launch_bounds__(256)
+spread_kernel(SpreadArgs a, const int4* __restrict__ work, float2* scratch) {
+  __shared__ float2 acc[SP_SB][SP_T * SP_T];
+  __shared__ int seg_pj[SP_LIST];
+  __shared__ int seg_off[SP_LIST + 1];
+  __shared__ int wsum[4];
+  __shared__ int nseg;
+  const int tid = threadIdx.x;
+  const int4 item = work[xcd_remap(blockIdx.x, gridDim.x)];
+  const int s0 = blockIdx.y * SP_SB;
+  const int ns = min(SP_SB, a.nsl - s0);
+  const TileWindow w = tile_window(a, item.x);
   for (int i = tid; i < SP_SB * SP_T * SP_T; i += 256)
     (&acc[0][0])[i] = make_float2(0.f, 0.f);
   const double hw = 0.5 * NUFFT_W;
-  const double lox = X0 - hw, hix = X1 - 1 + hw;
-  const double loy = Y0 - hw, hiy = Y1 - 1 + hw;
-  const int la = lane >> 3, lb = lane & 7;
   const long sstride = (long)a.P * a.N;
-
+  int base = 0;
   for (int ky = -1; ky <= 1; ++ky) {
     for (int kx = -1; kx <= 1; ++kx) {
       const double U0 = 0.5 * a.nf + kx * (double)a.nf;
       const double V0 = 0.5 * a.nf + ky * (double)a.nf;
-      if (U0 + a.umax <= lox || U0 - a.umax > hix || V0 + a.umax <= loy ||
-          V0 - a.umax > hiy)
-        continue;  // uniform over the workgroup
-      __syncthreads();
-      if (tid == 0) nseg = 0;
-      __syncthreads();
-      for (int p = tid; p < a.P; p += 256) {
-        double tlo = -1e30, thi = 1e30;
-        ray_interval(a.ax[p], U0, lox, hix, tlo, thi);
-        ray_interval(a.ay[p], V0, loy, hiy, tlo, thi);
-        if (tlo <= thi) {
-          tlo = fmax(tlo, -(double)a.N - 2.0);
-          thi = fmin(thi, (double)a.N + 2.0);
-          int jlo = max(0, (int)floor(tlo) + a.cidx - 1);
-          int jhi = min(a.N - 1, (int)ceil(thi) + a.cidx + 1);
-          if (jlo <= jhi) {
-            int k = atomicAdd(&nseg, 1);
-            if (k < SP_LIST) seg[k] = make_int2(p, jlo | ((jhi - jlo) << 16));
-          }
+      if (shift_skipped(a, w, U0, V0)) continue;
+      if (base >= item.z) continue;  // uniform: chunk already complete
+      int nsg;
+      const int total = enumerate_segments(a, U0, V0, w.lox, w.hix, w.loy,
+                                           w.hiy, seg_pj, seg_off, wsum,
+                                           &nseg, &nsg);
+      const int fa = max(item.y, base) - base;
+      const int fb = min(item.z, base + total) - base;
+      for (int f = fa + tid; f < fb; f += 256) {
+        int lo = 0, hi = nsg;  // seg_off[lo] <= f < seg_off[hi]
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (seg_off[mid] <= f) lo = mid; else hi = mid;
         }
-      }
-      __syncthreads();
-      const int nsg = min(nseg, SP_LIST);
-      for (int e = wid; e < nsg; e += 4) {
-        const int2 sg = seg[e];
-        const int p = sg.x;
-        const int jlo = sg.y & 0xffff;
-        const int jend = jlo + (sg.y >> 16) + 1;
-        const double axp = a.ax[p], ayp = a.ay[p];
-        const float2* cp = a.c + (long)p * a.N;
-        for (int j = jlo; j < jend; ++j) {
-          const double t = (double)(j - a.cidx);
-          const double u = U0 + t * axp;
-          const double v = V0 + t * ayp;
-          const int cx = (int)ceil(u - hw) + lb;
-          const int cy = (int)ceil(v - hw) + la;
-          if (cx >= X0 && cx < X1 && cy >= Y0 && cy < Y1) {
-            const float zx = (float)((cx - u) * (2.0 / NUFFT_W));
-            const float zy = (float)((cy - v) * (2.0 / NUFFT_W));
-            const float wgt = es_kernel(zx, a.beta) * es_kernel(zy, a.beta);
-            const int cell = (cy - Y0) * SP_T + (cx - X0);
-            for (int s = 0; s < ns; ++s) {
-              const float2 cv = cp[(long)(s0 + s) * sstride + j];
-              atomicAdd(&acc[s][cell].x, cv.x * wgt);
-              atomicAdd(&acc[s][cell].y, cv.y * wgt);
+        const int pj = seg_pj[lo];
+        const int p = pj & 2047;
+        const int j = (pj >> 11) + (f - seg_off[lo]);
+        const double t = (double)(j - a.cidx);
+        const double u = U0 + t * a.ax[p];
+        const double v = V0 + t * a.ay[p];
+        const int l0x = (int)ceil(u - hw);
+        const int l0y = (int)ceil(v - hw);
+        if (l0x + NUFFT_W <= w.X0 || l0x >= w.X1 || l0y + NUFFT_W <= w.Y0 ||
+            l0y >= w.Y1)
+          continue;
+        float wx[NUFFT_W], wy[NUFFT_W];
+#pragma unroll
+        for (int q = 0; q < NUFFT_W; ++q) {
+          wx[q] = es_kernel((float)((l0x + q - u) * (2.0 / NUFFT_W)), a.beta);
+          wy[q] = es_kernel((float)((l0y + q - v) * (2.0 / NUFFT_W)), a.beta);
+        }
+        float2 cv[SP_SB];
+        const float2* cp = a.c + (long)p * a.N + j;
+#pragma unroll
+        for (int s = 0; s < SP_SB; ++s)
+          cv[s] = s < ns ? cp[(long)(s0 + s) * sstride] : make_float2(0.f, 0.f);
+#pragma unroll
+        for (int qa = 0; qa < NUFFT_W; ++qa) {
+          const int cy = l0y + qa;
+          if (cy < w.Y0 || cy >= w.Y1) continue;
+#pragma unroll
+          for (int qb = 0; qb < NUFFT_W; ++qb) {
+            const int cx = l0x + qb;
+            if (cx < w.X0 || cx >= w.X1) continue;
+            const float wgt = wy[qa] * wx[qb];
+            const int cell = (cy - w.Y0) * SP_T + (cx - w.X0);
+#pragma unroll
+            for (int s = 0; s < SP_SB; ++s) {
+              if (s < ns) {
+                atomicAdd(&acc[s][cell].x, cv[s].x * wgt);
+                atomicAdd(&acc[s][cell].y, cv[s].y * wgt);
+              }
             }
           }
         }
       }
+      base += total;
     }
   }
   __syncthreads();
-  for (int i = tid; i < ns * SP_T * SP_T; i += 256) {
+  if (item.w < 0) {
+    for (int i = tid; i < ns * SP_T * SP_T; i += 256) {
+      const int s = i / (SP_T * SP_T);
+      const int cell = i - s * SP_T * SP_T;
+      const int ly = cell / SP_T, lx = cell - ly * SP_T;
+      if (w.Y0 + ly < w.Y1 && w.X0 + lx < w.X1)
+        a.fine[((long)(s0 + s) * a.nf + w.Y0 + ly) * a.nf + w.X0 + lx] =
+            acc[s][cell];
+    }
+  } else {
+    float2* dst = scratch + ((long)item.w * gridDim.y + blockIdx.y) *
+                                (SP_SB * SP_T * SP_T);
+    for (int i = tid; i < ns * SP_T * SP_T; i += 256) dst[i] = (&acc[0][0])[i];
+  }
+}
+
+// Adds the scratch slots of multi-chunk tiles into the fine grid, in slot
+// order (deterministic).
+__global__ void __launch_bounds__(256)
+spread_fixup_kernel(SpreadArgs a, const int4* __restrict__ fix,
+                    const float2* __restrict__ scratch) {
+  const int4 f = fix[blockIdx.x];
+  const int s0 = blockIdx.y * SP_SB;
+  const int ns = min(SP_SB, a.nsl - s0);
+  const int ty = f.x / a.ntx, tx = f.x - (f.x / a.ntx) * a.ntx;
+  const int X0 = tx * SP_T, Y0 = ty * SP_T;
+  const int X1 = min(X0 + SP_T, a.nf), Y1 = min(Y0 + SP_T, a.nf);
+  for (int i = threadIdx.x; i < ns * SP_T * SP_T; i += 256) {
     const int s = i / (SP_T * SP_T);
     const int cell = i - s * SP_T * SP_T;
     const int ly = cell / SP_T, lx = cell - ly * SP_T;
-    if (Y0 + ly < Y1 && X0 + lx < X1)
-      a.fine[((long)(s0 + s) * a.nf + Y0 + ly) * a.nf + X0 + lx] = acc[s][cell];
+    if (Y0 + ly >= Y1 || X0 + lx >= X1) continue;
+    float2 sum = make_float2(0.f, 0.f);
+    for (int k = 0; k < f.z; ++k) {
+      const float2 q = scratch[((long)(f.y + k) * gridDim.y + blockIdx.y) *
+                                   (SP_SB * SP_T * SP_T) + i];
+      sum.x += q.x;
+      sum.y += q.y;
+    }
+    float2* o = a.fine + ((long)(s0 + s) * a.nf + Y0 + ly) * a.nf + X0 + lx;
+    float2 cur = *o;
+    *o = make_float2(cur.x + sum.x, cur.y + sum.y);
   }
 }
 
@@ -337,6 +516,55 @@ float2* PolarNufft::fine(int batch) {
   return fine_.as<float2>();
 }
 
+void PolarNufft::build_work(hipStream_t stream) {
+  if (work_ready_) return;
+  const int ntx = (int)cdiv(nf_, SP_T);
+  const int ntiles = ntx * ntx;
+  const int cidx = N_ / 2;
+  SpreadArgs a{};
+  a.ax = ax_.as<double>();
+  a.ay = ay_.as<double>();
+  a.P = P_;
+  a.N = N_;
+  a.cidx = cidx;
+  a.nf = nf_;
+  a.ntx = ntx;
+  a.umax = std::max(cidx, N_ - 1 - cidx) * ((double)nf_ / (N_ - 1)) + 1.0;
+  DevBuf dcounts(sizeof(int) * ntiles);
+  hipLaunchKernelGGL(spread_count_kernel, dim3(ntiles), dim3(256), 0, stream,
+                     a, dcounts.as<int>());
+  TC_HIP(hipGetLastError());
+  std::vector<int> counts(ntiles);
+  TC_HIP(hipMemcpyAsync(counts.data(), dcounts.as<void>(), sizeof(int) * ntiles,
+                        hipMemcpyDeviceToHost, stream));
+  TC_HIP(hipStreamSynchronize(stream));
+  std::vector<int4> work, fix;
+  int slot = 0;
+  for (int t = 0; t < ntiles; ++t) {
+    const int c = counts[t];
+    const int nch = std::max(1, (c + SPREAD_CHUNK - 1) / SPREAD_CHUNK);
+    work.push_back(make_int4(t, 0, std::min(c, SPREAD_CHUNK), -1));
+    if (nch > 1) {
+      fix.push_back(make_int4(t, slot, nch - 1, 0));
+      for (int k = 1; k < nch; ++k)
+        work.push_back(make_int4(t, k * SPREAD_CHUNK,
+                                 std::min(c, (k + 1) * SPREAD_CHUNK), slot++));
+    }
+  }
+  nwork_ = (int)work.size();
+  nslots_ = slot;
+  nfix_ = (int)fix.size();
+  work_.alloc(sizeof(int4) * nwork_);
+  TC_HIP(hipMemcpy(work_.as<void>(), work.data(), sizeof(int4) * nwork_,
+                   hipMemcpyHostToDevice));
+  if (nfix_) {
+    fix_.alloc(sizeof(int4) * nfix_);
+    TC_HIP(hipMemcpy(fix_.as<void>(), fix.data(), sizeof(int4) * nfix_,
+                     hipMemcpyHostToDevice));
+  }
+  work_ready_ = true;
+}
+
 static int fine_batch(int nf, int S) {
   const double per = 8.0 * nf * (double)nf;
   int b = (int)std::max(1.0, std::floor(1.0e9 / per));
@@ -346,9 +574,13 @@ static int fine_batch(int nf, int S) {
 void PolarNufft::type1_real(const float2* c, float* out, int S, float scale,
                             hipStream_t stream) {
   TC_REQUIRE(S >= 1, "NUFFT: need at least one slice");
+  build_work(stream);
   const int B = fine_batch(nf_, S);
   float2* fg = fine(B);
   const int ntx = (int)cdiv(nf_, SP_T);
+  const unsigned ngroups_max = cdiv(B, SP_SB);
+  DevBuf scratch(sizeof(float2) * (size_t)std::max(nslots_, 1) * ngroups_max *
+                 SP_SB * SP_T * SP_T);
   const int cidx = N_ / 2;
   const double umax =
       std::max(cidx, N_ - 1 - cidx) * ((double)nf_ / (N_ - 1)) + 1.0;
@@ -367,9 +599,16 @@ void PolarNufft::type1_real(const float2* c, float* out, int S, float scale,
     a.ntx = ntx;
     a.umax = umax;
     a.beta = beta_;
-    dim3 grid((unsigned)(ntx * ntx), cdiv(nb, SP_SB));
-    hipLaunchKernelGGL(spread_kernel, grid, dim3(256), 0, stream, a);
+    const unsigned ng = cdiv(nb, SP_SB);
+    hipLaunchKernelGGL(spread_kernel, dim3((unsigned)nwork_, ng), dim3(256), 0,
+                       stream, a, work_.as<int4>(), scratch.as<float2>());
     TC_HIP(hipGetLastError());
+    if (nfix_) {
+      hipLaunchKernelGGL(spread_fixup_kernel, dim3((unsigned)nfix_, ng),
+                         dim3(256), 0, stream, a, fix_.as<int4>(),
+                         scratch.as<float2>());
+      TC_HIP(hipGetLastError());
+    }
     fft2d(fg, nb, nf_, nf_, nf_, (long)nf_ * nf_, +1, stream);
     const long tot = (long)nb * nm_ * nm_;
     hipLaunchKernelGGL(deconv_real_kernel, dim3(cdiv(tot, 256)), dim3(256), 0,
@@ -377,6 +616,8 @@ void PolarNufft::type1_real(const float2* c, float* out, int S, float scale,
                        nm_, nf_, nb, scale);
     TC_HIP(hipGetLastError());
   }
+  // the scratch slots are released on return
+  TC_HIP(hipStreamSynchronize(stream));
 }
 
 void PolarNufft::type2_real(const float* f, float2* c, int S,

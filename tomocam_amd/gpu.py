@@ -304,19 +304,27 @@ def pad2d(img: torch.Tensor, padding: int) -> torch.Tensor:
 
 def mbir(sino: torch.Tensor, angles, center: float, num_iters: int = 100,
          sigma: float = 100.0, tol: float = 1e-4, xtol: float = 1e-4,
-         refresh_every: int = 10, verbose: bool = False):
+         refresh_every: int = 10, verbose: bool = False,
+         guess: Optional[torch.Tensor] = None):
     """tomocam::mbir2 on a device-resident sinogram [S, P, N] -> [S, N, N].
 
-    Returns (recon, info) with info = lipschitz, step, sino_norm, max_sino,
-    padded width and accepted iterations."""
+    ``guess`` ([S, N, N]) replaces the FBP initial guess.  Returns (recon,
+    info) with info = lipschitz, step, sino_norm, max_sino, padded width and
+    accepted iterations."""
     sino = _as3d(sino)
     S, P, N = sino.shape
     ang = _angles(angles)
     if ang.size != P:
         raise ValueError(f"got {ang.size} angles for {P} projections")
+    x0 = None
+    if guess is not None:
+        guess = _as3d(guess)
+        if tuple(guess.shape) != (S, N, N):
+            raise ValueError(f"guess must be [{S}, {N}, {N}]")
+        x0 = _dev(guess, "guess")
     out = torch.empty((S, N, N), dtype=torch.float32, device=sino.device)
     info = np.zeros(6, dtype=np.float64)
-    _lib.call("tc_mbir2", _dev(sino, "sino"), _dev(out, "recon"), S, P, N,
+    _lib.call("tc_mbir2", _dev(sino, "sino"), x0, _dev(out, "recon"), S, P, N,
               ang.ctypes.data_as(C.c_void_p), float(center), int(num_iters),
               float(sigma), float(tol), float(xtol), int(refresh_every),
               int(bool(verbose)), info.ctypes.data_as(_lib.c_double_p),
