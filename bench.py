@@ -92,6 +92,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ktimer", action="store_true",
                     help="do not bracket kernels with HIP events")
+    ap.add_argument("--iterations", type=int, default=None,
+                    help="override the workload's iteration count (for "
+                         "short counter-collection runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 1 GiB device-to-device copy, whose "
+                         "FETCH_SIZE / WRITE_SIZE calibrate HBM counters")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -103,8 +109,19 @@ def main():
     import torch
     from tomocam_amd import gpu, phantom, _lib
     _lib.lib()
-    w = WORKLOADS[args.workload]
+    w = dict(WORKLOADS[args.workload])
+    if args.iterations is not None:
+        w["iterations"] = args.iterations
+        w["label"] += f" (iterations overridden to {args.iterations})"
     log(f"workload {args.workload}: {w['label']}")
+    if args.copy_calibration:
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+        c = torch.empty_like(a)
+        torch.cuda.synchronize()
+        torch.add(a, 1.0, out=c)  # elementwise kernel: 1 GiB in, 1 GiB out
+        torch.cuda.synchronize()
+        log("copy calibration: 1 GiB read + 1 GiB written")
+        del a, c
     sino, ang = make_sinogram(torch, gpu, phantom, w)
     center = w["npix"] / 2.0
 

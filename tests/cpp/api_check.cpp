@@ -66,9 +66,13 @@ int main(int argc, char** argv) {
     std::vector<tomocam::PointSpreadFunction<float>> psfs;
     psfs.emplace_back(grid);
     psfs[0].create_plans(4);
-    auto g2 = tomocam::gradient2(img, bp, psfs);
+    // at a second image (at img itself T img - bp cancels to ~1e-3)
+    tomocam::DArray<float> x2(img.dims());
+    for (size_t i = 0; i < x2.size(); ++i) x2[i] = (float)rng.rand();
+    dump(dir, "x2.f32", x2.begin(), x2.size());
+    auto g2 = tomocam::gradient2(x2, bp, psfs);
     dump(dir, "grad2.f32", g2.begin(), g2.size());
-    const float fv2 = tomocam::function_value2(img, bp, psfs, sino.norm());
+    const float fv2 = tomocam::function_value2(x2, bp, psfs, sino.norm());
     dump(dir, "fv2.f32", &fv2, 1);
 
     // mbir2 without a guess (the Python path)

@@ -107,3 +107,63 @@ def test_run2_divergent_sigma_first_iterations(torch_cuda, small_setup):
     out, err, _, _ = gpu_run2(torch_cuda, st, st.x0, 5, st.step, 0.1)
     assert rel_l2(out.cpu().numpy(), ref) < 1e-4
     np.testing.assert_allclose(err, log.errors, rtol=1e-4)
+
+
+
+
+def gaussian_psf(n):
+    """A Gaussian-sum PSF (positive spectrum: T is positive definite),
+    scaled so that max(T 1) is about 1."""
+    m = np.arange(-(n - 1), n, dtype=np.float64)
+    r2 = m[:, None] ** 2 + m[None, :] ** 2
+    ps = np.exp(-r2 / (2 * 2.5 ** 2)) + 0.02 * np.exp(-r2 / (2 * 60.0 ** 2))
+    return ps * (n ** 3 / ps.sum())
+
+
+# (n, slices, step * L, retries).  At these widths the FFT length has
+# compile-time plans and the step is committed inside the Toeplitz passes.
+# The optimum is 0 (b = 0, ||y||^2 = 1e-4), so no dot product cancels, and
+# the ratios (fx - fy) / gy of all attempts stay >= 36 % away from the
+# accept threshold 1: FP32 cannot flip a decision.
+@pytest.mark.parametrize("n,S,mult,retries", [(362, 2, 7.0, 2),
+                                              (1454, 1, 4.5, 1)])
+def test_run2_fused_commit_with_retries(torch_cuda, n, S, mult, retries):
+    """x0 = 1e-3 cos x cos y with a long step: the reference rejects the
+    first candidates (retries with x != sol), then accepts.  The fused
+    passes must also match the separate commit kernel bit for bit."""
+    import os
+    from tomocam_amd import gpu
+    sigma, iters, sino_norm = 1e4, 3, 1e-4
+    ps = gaussian_psf(n)
+    L = float(np.max(O.toeplitz(np.ones((1, n, n)), ps)))
+    u = (np.arange(n) + 0.5) / n
+    x0 = 1e-3 * (np.cos(np.pi * u)[None, :, None] *
+                 np.cos(np.pi * u)[None, None, :]) * np.ones((S, 1, 1))
+    b = np.zeros_like(x0)
+    step = mult / L
+    log = O.Run2Log()
+    ref = O.run2_linear(x0, lambda v: O.toeplitz(v, ps), b,
+                        lambda v: O.qggmrf_grad(v, sigma, 1.2), sino_norm,
+                        iters, step, log)
+    assert log.backtracks == retries
+    announce(f"run2 fused commit n={n} S={S} ({retries} retries)")
+    t = torch_cuda
+    psf = gpu.PointSpreadFunction(
+        psf_array=t.tensor(ps, dtype=t.float32).cuda())
+    x0t = t.tensor(x0, dtype=t.float32).cuda()
+    bt = t.tensor(b, dtype=t.float32).cuda()
+    out, err, _, stats = gpu.run2(psf, x0t, bt, iters, step, sigma, 1.2,
+                                  sino_norm)
+    assert stats["backtracks"] == retries
+    assert stats["attempts"] == iters + retries
+    assert rel_l2(out.cpu().numpy(), ref) < 1e-4
+    np.testing.assert_allclose(err, log.errors, rtol=1e-4)
+    os.environ["TOMOCAM_HIP_FUSED_COMMIT"] = "0"
+    try:
+        out2, err2, _, stats2 = gpu.run2(psf, x0t, bt, iters, step, sigma,
+                                         1.2, sino_norm)
+    finally:
+        del os.environ["TOMOCAM_HIP_FUSED_COMMIT"]
+    assert stats2["backtracks"] == retries
+    assert t.equal(out, out2)
+    np.testing.assert_array_equal(err, err2)

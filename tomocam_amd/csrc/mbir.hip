@@ -1,6 +1,7 @@
 // qGGMRF stencil, fused MBIR gradient/commit kernels and the run2 driver.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <utility>
 
 #include "lds_launch.h"
@@ -28,12 +29,13 @@ struct RhoParams {
   float qmp;  // q - p
 };
 
-__device__ __forceinline__ float drho(float d, const RhoParams& P) {
+// rho'(d) * w for a neighbour of weight w (ws2 = w / sigma^2)
+__device__ __forceinline__ float wdrho(float d, float ws2, const RhoParams& P) {
   const float a = fabsf(d) * P.inv_sigma;
   // (|d|/sigma)^(q-p) = 2^((q-p) log2(|d|/sigma)); log2(0) = -inf -> t = 0
   const float t = __builtin_amdgcn_exp2f(P.qmp * __builtin_amdgcn_logf(a));
   const float r = __builtin_amdgcn_rcpf(MRF_C + t);
-  return d * P.inv_sigma2 * r * (2.0f - P.qmp * t * r);
+  return d * ws2 * r * (2.0f - P.qmp * t * r);
 }
 
 struct QArgs {
@@ -48,115 +50,243 @@ struct QArgs {
   int zchunk;
   float beta;
   RhoParams rp;
+  float ws2_face, ws2_edge, ws2_corner;
 };
 
-constexpr int QTX = 64, QTY = 8, QPW = QTX + 2, QPH = QTY + 2;
+// Pair-symmetric stencil.  rho' is odd, so the term a voxel a receives
+// from neighbour a + o is minus the term a + o receives from a:
+//   g(a) = sum_{o in F} phi_o(a) - sum_{o in F} phi_o(a - o),
+//   phi_o(p) = w_o rho'(V(p) - V(p + o)),
+// with F the 13 "forward" offsets (dz = +1: 9; dz = 0, dy = +1: 3;
+// dz = dy = 0, dx = +1: 1) and V the volume extended by replication.  The
+// extension makes the identity exact at the faces too: for a in the volume
+// and a - o outside, w rho'(v_a - V(a - o)) = -phi_o(a - o) because
+// V(a - o + o) = v_a, which is the reference's clamped-neighbour rule.
+//
+// Layout: one wave holds a strip of QROWS output rows and QCOLS = 62 output
+// columns; lane l is column c0 - 1 + l (lanes 0 and 63 are the x halo) and
+// each lane keeps its column's QROWS + 2 rows (y halo) of two planes in
+// registers.  x neighbours come from the adjacent lane (DPP wave shift), y
+// neighbours from the lane's own registers, and the wave marches in z,
+// carrying the backward terms of the dz = +1 pairs to the next plane.
+// Each pair is evaluated once: 13 QROWS + 9 rho' per strip row-plane
+// instead of 26 QROWS.
+constexpr int QLANES = 64;
+constexpr int QCOLS = QLANES - 2;
+constexpr int QROWS = 8;
+constexpr int QWAVES = 4;  // strips per workgroup, stacked in y
+constexpr int QJ = QROWS + 2;
 // line-search dot products per workgroup (MBIR mode):
 //   y.Ty, y.b, g.(Ty - b), g.g, d.d, d.g   (d = sol - x)
 constexpr int NDOT = 6;
 
+// value held by lane - 1 (column x - 1) / lane + 1 (column x + 1); the end
+// lanes receive 0 and only feed halo lanes, whose results are discarded.
+// Plain DPP moves (bound_ctrl, full masks) fold into the consuming VALU op.
+__device__ __forceinline__ float from_left(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v),
+                                      0x138 /* wave_shr:1 */, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float from_right(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v),
+                                      0x130 /* wave_shl:1 */, 0xf, 0xf, true));
+}
+
+template <int DX>
+__device__ __forceinline__ float shift_x(float v) {
+  if constexpr (DX > 0) return from_left(v);
+  else if constexpr (DX < 0) return from_right(v);
+  else return v;
+}
+
+// The pair terms of forward offset (DZ, DY, DX) between plane `lo` and
+// plane `hi` (= lo for DZ = 0).  FWD: add phi(a) to glo for the strip's
+// rows; always subtract phi(a - o) from ghi.
+template <int DY, int DX, bool FWD>
+__device__ __forceinline__ void qpair(const float (&lo)[QJ],
+                                      const float (&hi)[QJ], float (&glo)[QJ],
+                                      float (&ghi)[QJ], float ws2,
+                                      const RhoParams& P) {
+  constexpr int jlo = FWD ? (DY > 0 ? 0 : 1) : 1 - DY;
+  constexpr int jhi = FWD ? (DY < 0 ? QROWS + 1 : QROWS) : QROWS - DY;
+  float phi[QJ];
+#pragma unroll
+  for (int j = jlo; j <= jhi; ++j) {
+    // V(a + o): column x + DX of plane hi
+    phi[j] = wdrho(lo[j] - shift_x<-DX>(hi[j + DY]), ws2, P);
+  }
+  if constexpr (FWD) {
+#pragma unroll
+    for (int j = 1; j <= QROWS; ++j) glo[j] += phi[j];
+  }
+#pragma unroll
+  for (int j = 1; j <= QROWS; ++j) ghi[j] -= shift_x<DX>(phi[j - DY]);
+  // Materialise the accumulators after each pair set.  Without this the
+  // compiler defers the additions (to the output branch or the next plane)
+  // and keeps every term live: > 256 VGPRs.  8-9 independent rho' chains
+  // per set are enough to hide the latency.
+#pragma unroll
+  for (int j = 1; j <= QROWS; ++j) {
+    if constexpr (FWD) asm volatile("" : "+v"(glo[j]));
+    asm volatile("" : "+v"(ghi[j]));
+  }
+}
+
+// the 9 dz = +1 pairs between planes lo and hi
+template <bool FWD>
+__device__ __forceinline__ void qcross(const float (&lo)[QJ],
+                                       const float (&hi)[QJ],
+                                       float (&glo)[QJ], float (&ghi)[QJ],
+                                       const QArgs& a) {
+  qpair<0, 0, FWD>(lo, hi, glo, ghi, a.ws2_face, a.rp);
+  qpair<1, 0, FWD>(lo, hi, glo, ghi, a.ws2_edge, a.rp);
+  qpair<-1, 0, FWD>(lo, hi, glo, ghi, a.ws2_edge, a.rp);
+  qpair<0, 1, FWD>(lo, hi, glo, ghi, a.ws2_edge, a.rp);
+  qpair<0, -1, FWD>(lo, hi, glo, ghi, a.ws2_edge, a.rp);
+  qpair<1, 1, FWD>(lo, hi, glo, ghi, a.ws2_corner, a.rp);
+  qpair<1, -1, FWD>(lo, hi, glo, ghi, a.ws2_corner, a.rp);
+  qpair<-1, 1, FWD>(lo, hi, glo, ghi, a.ws2_corner, a.rp);
+  qpair<-1, -1, FWD>(lo, hi, glo, ghi, a.ws2_corner, a.rp);
+}
+
+// the 4 in-plane forward pairs: (0,0,+1), (0,+1,-1), (0,+1,0), (0,+1,+1)
+__device__ __forceinline__ void qplane(const float (&v)[QJ], float (&g)[QJ],
+                                       const QArgs& a) {
+  qpair<0, 1, true>(v, v, g, g, a.ws2_face, a.rp);
+  qpair<1, 0, true>(v, v, g, g, a.ws2_face, a.rp);
+  qpair<1, 1, true>(v, v, g, g, a.ws2_edge, a.rp);
+  qpair<1, -1, true>(v, v, g, g, a.ws2_edge, a.rp);
+}
+
+// y (= sol, or sol + beta (sol - xprev)) of plane z at the strip's
+// in-plane offsets ro (rows r0 - 1 .. r0 + QROWS, replicated at the faces).
+// Plane pointers are wave-uniform and offsets 32-bit, so every load is
+// scalar base + one VGPR.
+template <typename T>
+__device__ __forceinline__ T& at_byte(T* base, unsigned off) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off);
+}
+template <typename T>
+__device__ __forceinline__ const T& at_byte(const T* base, unsigned off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                     off);
+}
+
+template <bool HAS_X>
+__device__ __forceinline__ void qload(float (&v)[QJ], const QArgs& a, int z,
+                                      const unsigned (&ro)[QJ]) {
+  const long zo = (long)min(max(z, 0), a.S - 1) * a.R * a.C;
+  const float* sp = a.sol + zo;
+  const float* xp = a.xprev + zo;
+#pragma unroll
+  for (int j = 0; j < QJ; ++j) {
+    float s = at_byte(sp, ro[j]);
+    if constexpr (HAS_X) s = fmaf(a.beta, s - at_byte(xp, ro[j]), s);
+    v[j] = s;
+  }
+}
+
 // MODE 0: g += R'(x).  MODE 1 (MBIR): y = sol + beta (sol - xprev),
 // g = Tsol + beta (Tsol - Tx) - b + R'(y), plus the line-search dots.
-template <int MODE>
+// HAS_X = false means xprev == sol and Tx == Tsol (y = sol, d = 0).
+template <int MODE, bool HAS_X>
 __global__ void __launch_bounds__(256) qggmrf_kernel(QArgs a) {
-  __shared__ float pl[3][QPH * QPW];
   __shared__ double red[16];
-  const int tid = threadIdx.x;
-  const int ntx = (a.C + QTX - 1) / QTX;
-  const int nty = (a.R + QTY - 1) / QTY;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int ntx = (a.C + QCOLS - 1) / QCOLS;
+  const int nty = (a.R + QROWS * QWAVES - 1) / (QROWS * QWAVES);
   const int ntiles = ntx * nty;
   const int bl = xcd_remap(blockIdx.x, gridDim.x);
   const int zc = bl / ntiles;
   const int tile = bl - zc * ntiles;
   const int ty = tile / ntx;
   const int tx = tile - ty * ntx;
-  const int r0 = ty * QTY, c0 = tx * QTX;
+  const int x = tx * QCOLS - 1 + lane;
+  const int xc = min(max(x, 0), a.C - 1);
+  const int r0 = (ty * QWAVES + wave) * QROWS;
   const int z0 = zc * a.zchunk;
   const int z1 = min(a.S, z0 + a.zchunk);
-  const int lx = tid & 63;
-  const int ly = tid >> 6;
+  const bool xout = lane >= 1 && lane <= QCOLS && x < a.C;
   const long plane = (long)a.R * a.C;
+  // in-plane byte offsets of the strip's rows (planes hold < 2^30
+  // elements); for output rows (r < R) with x < C they are the outputs'
+  // offsets too
+  unsigned ro[QJ];
+#pragma unroll
+  for (int j = 0; j < QJ; ++j)
+    ro[j] = 4u * (unsigned)(min(max(r0 - 1 + j, 0), a.R - 1) * a.C + xc);
 
-  auto load = [&](int slot, int z) {
-    const int zz = min(max(z, 0), a.S - 1);
-    const long zo = (long)zz * plane;
-    for (int t = tid; t < QPH * QPW; t += 256) {
-      int rr = t / QPW;
-      int cc = t - rr * QPW;
-      int r = min(max(r0 + rr - 1, 0), a.R - 1);
-      int c = min(max(c0 + cc - 1, 0), a.C - 1);
-      long o = zo + (long)r * a.C + c;
-      float v = a.sol[o];
-      if (MODE == 1) v = fmaf(a.beta, v - a.xprev[o], v);
-      pl[slot][t] = v;
-    }
-  };
+  float vc[QJ], vn[QJ];
+  float gc[QJ], gn[QJ];
+#pragma unroll
+  for (int j = 0; j < QJ; ++j) gc[j] = 0.f;
 
-  int sm1 = 0, s0 = 1, sp1 = 2;
-  load(sm1, z0 - 1);
-  load(s0, z0);
-  load(sp1, z0 + 1);
-  __syncthreads();
+  // backward terms of plane z0 from the pairs (z0 - 1, z0)
+  qload<HAS_X>(vn, a, z0 - 1, ro);
+  qload<HAS_X>(vc, a, z0, ro);
+  qcross<false>(vn, vc, gn, gc, a);
 
   double acc[NDOT] = {0, 0, 0, 0, 0, 0};
   for (int z = z0; z < z1; ++z) {
-    const float* P0 = pl[sm1];
-    const float* P1 = pl[s0];
-    const float* P2 = pl[sp1];
+    // opaque per iteration, so the 32-bit offsets are not hoisted as 64-bit
+    // values and every access uses the scalar-base + VGPR-offset form
 #pragma unroll
-    for (int half = 0; half < 2; ++half) {
-      const int lr = ly + 4 * half;
-      const int r = r0 + lr, c = c0 + lx;
-      if (r < a.R && c < a.C) {
-        const int ctr = (lr + 1) * QPW + (lx + 1);
-        const float v = P1[ctr];
-        float sum = 0.f;
+    for (int j = 0; j < QJ; ++j) asm volatile("" : "+v"(ro[j]));
+    qload<HAS_X>(vn, a, z + 1, ro);
 #pragma unroll
-        for (int dz = 0; dz < 3; ++dz) {
-          const float* Pz = dz == 0 ? P0 : (dz == 1 ? P1 : P2);
+    for (int j = 0; j < QJ; ++j) gn[j] = 0.f;
+    qplane(vc, gc, a);
+    qcross<true>(vc, vn, gc, gn, a);
+
+    if (xout) {
+      const long zo = (long)z * plane;
+      float* gp = a.g + zo;
+      const float* bp = a.b + zo;
+      const float* tsp = a.Tsol + zo;
+      const float* txp = a.Tx + zo;
+      const float* sp = a.sol + zo;
+      const float* xpp = a.xprev + zo;
 #pragma unroll
-          for (int dy = 0; dy < 3; ++dy) {
-#pragma unroll
-            for (int dx = 0; dx < 3; ++dx) {
-              const int k = (dz != 1) + (dy != 1) + (dx != 1);
-              if (k == 0) continue;
-              const float w = k == 1 ? W_FACE : (k == 2 ? W_EDGE : W_CORNER);
-              sum += w * drho(v - Pz[ctr + (dy - 1) * QPW + (dx - 1)], a.rp);
+      for (int j = 1; j <= QROWS; ++j) {
+        if (r0 + j - 1 < a.R) {
+          const unsigned o = ro[j];
+          if constexpr (MODE == 0) {
+            at_byte(gp, o) += gc[j];
+          } else {
+            const float v = vc[j];
+            const float bb = at_byte(bp, o);
+            float tyv = at_byte(tsp, o);
+            if constexpr (HAS_X)
+              tyv = fmaf(a.beta, tyv - at_byte(txp, o), tyv);
+            const float gv = (tyv - bb) + gc[j];
+            at_byte(gp, o) = gv;
+            acc[0] += (double)v * tyv;
+            acc[1] += (double)v * bb;
+            acc[2] += (double)gv * (tyv - bb);
+            acc[3] += (double)gv * gv;
+            if constexpr (HAS_X) {
+              const float d = at_byte(sp, o) - at_byte(xpp, o);
+              acc[4] += (double)d * d;
+              acc[5] += (double)d * gv;
             }
           }
         }
-        const long o = (long)z * plane + (long)r * a.C + c;
-        if (MODE == 0) {
-          a.g[o] += sum;
-        } else {
-          const float so = a.sol[o], xp = a.xprev[o];
-          const float ts = a.Tsol[o], tq = a.Tx[o], bb = a.b[o];
-          const float tyv = fmaf(a.beta, ts - tq, ts);
-          const float gv = (tyv - bb) + sum;
-          a.g[o] = gv;
-          const float d = so - xp;
-          acc[0] += (double)v * tyv;
-          acc[1] += (double)v * bb;
-          acc[2] += (double)gv * (tyv - bb);
-          acc[3] += (double)gv * gv;
-          acc[4] += (double)d * d;
-          acc[5] += (double)d * gv;
-        }
       }
     }
-    __syncthreads();
-    const int t = sm1;
-    sm1 = s0;
-    s0 = sp1;
-    sp1 = t;
-    if (z + 1 < z1) load(sp1, z + 2);
-    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < QJ; ++j) {
+      vc[j] = vn[j];
+      gc[j] = gn[j];
+    }
   }
   if (MODE == 1) {
 #pragma unroll
     for (int k = 0; k < NDOT; ++k) {
       double tot = block_sum(acc[k], red);
-      if (tid == 0) a.partials[(long)bl * NDOT + k] = tot;
+      if (threadIdx.x == 0) a.partials[(long)bl * NDOT + k] = tot;
     }
   }
 }
@@ -170,13 +300,41 @@ static RhoParams make_rho(float sigma, float p) {
   return r;
 }
 
+static void set_rho(QArgs& a, float sigma, float p) {
+  a.rp = make_rho(sigma, p);
+  a.ws2_face = W_FACE * a.rp.inv_sigma2;
+  a.ws2_edge = W_EDGE * a.rp.inv_sigma2;
+  a.ws2_corner = W_CORNER * a.rp.inv_sigma2;
+}
+
+// z chunking: a chunk costs its planes plus ~0.7 plane for the pairs with
+// the plane before it; waves are spread over the SIMDs, so pick the chunk
+// that minimises (rounds of waves per SIMD) x (planes per wave).
 static void qggmrf_grid(int S, int R, int C, int* zchunk, unsigned* nblk) {
-  const int ntiles = (int)(cdiv(C, QTX) * cdiv(R, QTY));
-  int zc = S;
-  // keep at least ~2048 workgroups in flight
-  while (zc > 4 && (long)ntiles * cdiv(S, zc) < 2048) zc = (zc + 1) / 2;
-  *zchunk = zc;
-  *nblk = (unsigned)(ntiles * cdiv(S, zc));
+  static int simds = 0;
+  if (!simds) {
+    int dev = 0, cus = 0;
+    TC_HIP(hipGetDevice(&dev));
+    TC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                 dev));
+    simds = 4 * (cus > 0 ? cus : 256);
+  }
+  const long ntiles = cdiv(C, QCOLS) * cdiv(R, QROWS * QWAVES);
+  int best = S;
+  double best_cost = 1e300;
+  for (int zc = S; zc >= 1; --zc) {
+    const long nz = cdiv(S, zc);
+    if (zc != S && cdiv(S, zc + 1) == nz) continue;  // same count, longer
+    const long waves = ntiles * QWAVES * nz;
+    const double rounds = (double)cdiv(waves, simds);
+    const double cost = rounds * (zc + 0.7);
+    if (cost < best_cost * 0.999) {
+      best_cost = cost;
+      best = zc;
+    }
+  }
+  *zchunk = best;
+  *nblk = (unsigned)(ntiles * cdiv(S, best));
 }
 
 void add_total_var(const float* x, float* g, int S, int R, int C, float sigma,
@@ -188,11 +346,11 @@ void add_total_var(const float* x, float* g, int S, int R, int C, float sigma,
   a.S = S;
   a.R = R;
   a.C = C;
-  a.rp = make_rho(sigma, p);
+  set_rho(a, sigma, p);
   unsigned nblk;
   qggmrf_grid(S, R, C, &a.zchunk, &nblk);
   ktimer_begin(KC_QGGMRF, stream);
-  hipLaunchKernelGGL(qggmrf_kernel<0>, dim3(nblk), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL((qggmrf_kernel<0, false>), dim3(nblk), dim3(256), 0, stream, a);
   TC_HIP(hipGetLastError());
   ktimer_end(KC_QGGMRF, stream, 3.0 * 4.0 * S * (double)R * C);
 }
@@ -300,7 +458,7 @@ Run2Stats run2_device(Psf& psf, const float* x0, const float* b, float* out,
   qa.S = S;
   qa.R = n;
   qa.C = n;
-  qa.rp = make_rho(opt.sigma, opt.p);
+  set_rho(qa, opt.sigma, opt.p);
   unsigned qblk;
   qggmrf_grid(S, n, n, &qa.zchunk, &qblk);
   const int ntp = toeplitz_num_partials(psf, S);
@@ -320,6 +478,11 @@ Run2Stats run2_device(Psf& psf, const float* x0, const float* b, float* out,
   TC_HIP(hipMemcpyAsync(A, x0, vb, hipMemcpyDeviceToDevice, stream));
   psf.apply(A, nullptr, TA, S, 0, nullptr, stream);
   float *sol = A, *xp = A, *Tsol = TA, *Tx = TA;
+
+  // TOMOCAM_HIP_FUSED_COMMIT=0 selects the separate commit kernel (same
+  // arithmetic, one more pass over HBM)
+  const char* fenv = std::getenv("TOMOCAM_HIP_FUSED_COMMIT");
+  const bool allow_fused = !(fenv && fenv[0] == '0');
 
   Run2Stats st;
   st.toeplitz_applies = 1;
@@ -342,11 +505,27 @@ Run2Stats run2_device(Psf& psf, const float* x0, const float* b, float* out,
       qa.Tx = has_x ? Tx : Tsol;
       qa.beta = has_x ? (float)beta : 0.0f;
       ktimer_begin(KC_MBIR_GRADIENT, stream);
-      hipLaunchKernelGGL(qggmrf_kernel<1>, dim3(qblk), dim3(256), 0, stream, qa);
+      if (has_x)
+        hipLaunchKernelGGL((qggmrf_kernel<1, true>), dim3(qblk), dim3(256), 0,
+                           stream, qa);
+      else
+        hipLaunchKernelGGL((qggmrf_kernel<1, false>), dim3(qblk), dim3(256), 0,
+                           stream, qa);
       TC_HIP(hipGetLastError());
       // reads sol, T sol, b (and x, T x unless aliased), writes g
       ktimer_end(KC_MBIR_GRADIENT, stream, (has_x ? 6.0 : 4.0) * vb);
-      psf.apply(g, nullptr, Tg, S, 0, tparts, stream);
+      // The step is committed inside the Toeplitz passes where the FFT size
+      // allows: cand = y - s g and T cand = T y - s T g are written before
+      // the accept test, which then only decides the buffers' roles.  In
+      // place unless x aliases sol (then sol must survive a rejection).
+      float* fdst = has_x ? sol : (sol == A ? B : A);
+      float* fTdst = has_x ? Tsol : (Tsol == TA ? TB : TA);
+      const Psf::StepCommit sc{sol, has_x ? xp : nullptr, Tsol,
+                               has_x ? Tx : nullptr, fdst, fTdst,
+                               has_x ? (float)beta : 0.0f, (float)step};
+      const bool fused =
+          allow_fused && psf.apply_commit(g, sc, S, tparts, stream);
+      if (!fused) psf.apply(g, nullptr, Tg, S, 0, tparts, stream);
       ++st.toeplitz_applies;
       hipLaunchKernelGGL(reduce_strided_kernel, dim3(NDOT), dim3(1024), 0,
                          stream, qparts, (long)qblk, NDOT,
@@ -372,18 +551,22 @@ Run2Stats run2_device(Psf& psf, const float* x0, const float* b, float* out,
       // fx > fy + gy  <=>  (fx - fy) > gy, with fx - fy = dlt / (fx + fy);
       // NaN compares false (accept) as in the reference
       const bool reject = (dlt / (fx + fy)) > gy;
-      // cand = y - s g, T cand = T y - s T g; in place unless x aliases sol
-      // and must survive a rejection
-      float* dst = (reject && !has_x) ? (sol == A ? B : A) : sol;
-      float* Tdst = (reject && !has_x) ? (Tsol == TA ? TB : TA) : Tsol;
-      const float cb = has_x ? (float)beta : 0.0f;
-      ktimer_begin(KC_MBIR_COMMIT, stream);
-      hipLaunchKernelGGL(commit_kernel, dim3(cblk), dim3(256), 0, stream, sol,
-                         qa.xprev, Tsol, qa.Tx, g, Tg, dst, Tdst, N / 4, N, cb,
-                         (float)s);
-      TC_HIP(hipGetLastError());
-      // reads sol, T sol, g, T g (and x, T x unless aliased), writes 2
-      ktimer_end(KC_MBIR_COMMIT, stream, (has_x ? 8.0 : 6.0) * vb);
+      float* dst = fdst;
+      float* Tdst = fTdst;
+      if (!fused) {
+        // cand = y - s g, T cand = T y - s T g; in place unless x aliases
+        // sol and must survive a rejection
+        dst = (reject && !has_x) ? (sol == A ? B : A) : sol;
+        Tdst = (reject && !has_x) ? (Tsol == TA ? TB : TA) : Tsol;
+        const float cb = has_x ? (float)beta : 0.0f;
+        ktimer_begin(KC_MBIR_COMMIT, stream);
+        hipLaunchKernelGGL(commit_kernel, dim3(cblk), dim3(256), 0, stream,
+                           sol, qa.xprev, Tsol, qa.Tx, g, Tg, dst, Tdst, N / 4,
+                           N, cb, (float)s);
+        TC_HIP(hipGetLastError());
+        // reads sol, T sol, g, T g (and x, T x unless aliased), writes 2
+        ktimer_end(KC_MBIR_COMMIT, stream, (has_x ? 8.0 : 6.0) * vb);
+      }
       sol = dst;
       Tsol = Tdst;
       if (reject) {

@@ -36,6 +36,26 @@ class Psf {
   void apply(const float* x, const float* b, float* out, int nslices,
              int mode, double* dot_partials, hipStream_t stream);
 
+  // T g with a line-search step committed in the same passes (pass A reads
+  // g, pass C produces T g):
+  //   dst  = y - s g,      y  = sol + beta (sol - xp)
+  //   Tdst = T y - s T g,  Ty = Tsol + beta (Tsol - Tx)
+  // (xp == Tx == nullptr: y = sol, Ty = Tsol).  dst / Tdst may alias sol /
+  // Tsol; T g itself is not stored.  Returns false, launching nothing, when
+  // this FFT size has no fused passes.
+  struct StepCommit {
+    const float* sol;
+    const float* xp;
+    const float* Tsol;
+    const float* Tx;
+    float* dst;
+    float* Tdst;
+    float beta;
+    float s;
+  };
+  bool apply_commit(const float* g, const StepCommit& c, int nslices,
+                    double* dot_partials, hipStream_t stream);
+
   // Grows the workspace for `nslices` slices.
   void reserve(int nslices);
 

@@ -241,7 +241,22 @@ int toeplitz_ct_lpw(int M);
 bool toeplitz_apply_ct(int M, const float* x, const float* b, float* out,
                        float2* wt, const float* spec, int n, int H, int rp,
                        int nslices, int mode, double* partials,
-                       hipStream_t stream, double img_b, double spec_b);
+                       hipStream_t stream, double img_b, double spec_b,
+                       const Psf::StepCommit* cm);
+
+bool Psf::apply_commit(const float* g, const StepCommit& c, int nslices,
+                       double* dot_partials, hipStream_t stream) {
+  TC_REQUIRE(nslices >= 1, "Toeplitz: need at least one slice");
+  TC_REQUIRE((c.xp == nullptr) == (c.Tx == nullptr),
+             "Toeplitz: xp and Tx must both be given or both be null");
+  if (toeplitz_ct_lpw(M_) == 0) return false;
+  reserve(nslices);
+  const double img_b = 4.0 * nslices * (double)n_ * n_;
+  const double spec_b = 8.0 * nslices * (double)n_ * H_;
+  return toeplitz_apply_ct(M_, g, nullptr, c.Tdst, work_.as<float2>(),
+                           spectrum(), n_, H_, rp_, nslices, 0, dot_partials,
+                           stream, img_b, spec_b, &c);
+}
 
 void Psf::apply(const float* x, const float* b, float* out, int nslices,
                 int mode, double* dot_partials, hipStream_t stream) {
@@ -253,7 +268,7 @@ void Psf::apply(const float* x, const float* b, float* out, int nslices,
   const double img_b = 4.0 * nslices * (double)n_ * n_;
   const double spec_b = 8.0 * nslices * (double)n_ * H_;
   if (toeplitz_apply_ct(M_, x, b, out, wt, spectrum(), n_, H_, rp_, nslices,
-                        mode, dot_partials, stream, img_b, spec_b))
+                        mode, dot_partials, stream, img_b, spec_b, nullptr))
     return;
   const FFTLength& F = fft_length(M_);
   const int npairs = (n_ + 1) / 2;

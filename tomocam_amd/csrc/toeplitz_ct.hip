@@ -36,14 +36,37 @@ using Plan1458 = TPlan<1458, 18, 3, Radices<18, 9, 9>, Radices<9, 9, 18>>;
 using Plan2916 = TPlan<2916, 18, 3, Radices<18, 18, 9>, Radices<9, 18, 18>>;
 using Plan5832 = TPlan<5832, 18, 1, Radices<18, 18, 18>, Radices<18, 18, 18>>;
 
+// Line-search commit fused into passes A and C (CM = 0: none; 1: y = sol;
+// 2: y = sol + beta (sol - xp)).  Pointers may alias (in-place update).
+struct CommitPtrs {
+  const float* sol;
+  const float* xp;
+  float* dst;
+  const float* Tsol;
+  const float* Tx;
+  float* Tdst;
+  float beta;
+  float s;
+};
+
 // ---------------------------------------------------------------------------
 // pass A: pair-packed forward row FFTs -> Wt[s][k1][row]
+//   (+ dst = y - s x when committing a step: x is the gradient there)
 // ---------------------------------------------------------------------------
 
-template <class P>
+template <int CM>
+__device__ __forceinline__ void commit_elem(const CommitPtrs& c, long o,
+                                            float gv) {
+  float y = c.sol[o];
+  if constexpr (CM == 2) y = fmaf(c.beta, y - c.xp[o], y);
+  c.dst[o] = fmaf(-c.s, gv, y);
+}
+
+template <class P, int CM>
 __global__ void __launch_bounds__(P::NT)
 ct_rows_fwd(const float* __restrict__ x, float2* __restrict__ wt,
-            const float2* __restrict__ tw, int n, int H, int rp, int nbps) {
+            const float2* __restrict__ tw, int n, int H, int rp, int nbps,
+            CommitPtrs cp) {
   extern __shared__ float2 sm[];
   constexpr int L = P::L, E = P::E, T = P::T, LPW = P::LPW;
   using F = Xform<L, E, -1, typename P::Fwd>;
@@ -67,6 +90,11 @@ ct_rows_fwd(const float* __restrict__ x, float2* __restrict__ wt,
       if (c < n) {
         if (hasa) re = rowa[c];
         if (hasb) im = rowa[n + c];
+        if constexpr (CM != 0) {
+          const long o = (long)s * n * n + (long)ra * n + c;
+          if (hasa) commit_elem<CM>(cp, o, re);
+          if (hasb) commit_elem<CM>(cp, o + n, im);
+        }
       }
       v[i * R1 + r] = make_float2(re, im);
     }
@@ -162,13 +190,28 @@ ct_cols(float2* __restrict__ wt, const float* __restrict__ spec,
 
 // ---------------------------------------------------------------------------
 // pass C: Hermitian completion, pair-packed inverse row FFTs, crop, epilogue
+//   MODE 0: out = T x;  1: out = T x - b;
+//   2 / 3: out = Ty - s T x with Ty = Tsol (2) or Tsol + beta (Tsol - Tx) (3)
 // ---------------------------------------------------------------------------
+
+template <int MODE>
+__device__ __forceinline__ float rows_inv_epilogue(float q, const float* bvec,
+                                                   const CommitPtrs& c,
+                                                   long o) {
+  if constexpr (MODE == 0) return q;
+  else if constexpr (MODE == 1) return q - bvec[o];
+  else {
+    float ty = c.Tsol[o];
+    if constexpr (MODE == 3) ty = fmaf(c.beta, ty - c.Tx[o], ty);
+    return fmaf(-c.s, q, ty);
+  }
+}
 
 template <class P, int MODE>
 __global__ void __launch_bounds__(P::NT)
-ct_rows_inv(const float2* __restrict__ wt, const float* __restrict__ bvec,
-            float* __restrict__ out, const float2* __restrict__ tw, int n,
-            int H, int rp, int nbps) {
+ct_rows_inv(const float2* __restrict__ wt, const float* bvec, float* out,
+            const float2* __restrict__ tw, int n, int H, int rp, int nbps,
+            CommitPtrs cp) {
   extern __shared__ float2 sm[];
   constexpr int L = P::L, E = P::E, T = P::T, LPW = P::LPW;
   using I = Xform<L, E, 1, typename P::Inv>;
@@ -212,11 +255,11 @@ ct_rows_inv(const float2* __restrict__ wt, const float* __restrict__ bvec,
         const float2 q = v[i * RS + r];
         if (ra < n) {
           const long o = so + (long)ra * n + c;
-          out[o] = MODE == 1 ? q.x - bvec[o] : q.x;
+          out[o] = rows_inv_epilogue<MODE>(q.x, bvec, cp, o);
         }
         if (ra + 1 < n) {
           const long o = so + (long)(ra + 1) * n + c;
-          out[o] = MODE == 1 ? q.y - bvec[o] : q.y;
+          out[o] = rows_inv_epilogue<MODE>(q.y, bvec, cp, o);
         }
       }
     }
@@ -228,19 +271,33 @@ template <class P>
 static void run_ct(const float* x, const float* b, float* out, float2* wt,
                    const float* spec, int n, int H, int rp, int nslices,
                    int mode, double* partials, hipStream_t stream,
-                   double img_b, double spec_b) {
+                   double img_b, double spec_b, const Psf::StepCommit* sc) {
   const float2* tw = fft_length(P::L).plan().tw;
   const size_t smem = sizeof(float2) * P::LPW * P::L;
   const int npairs = (n + 1) / 2;
+  CommitPtrs cp{};
+  int cm = 0;
+  if (sc) {
+    cm = sc->xp ? 2 : 1;
+    cp = CommitPtrs{sc->sol, sc->xp, sc->dst, sc->Tsol, sc->Tx, sc->Tdst,
+                    sc->beta, sc->s};
+    out = sc->Tdst;
+  }
   {
     const int nbps = (int)cdiv(npairs, P::LPW);
-    auto k = ct_rows_fwd<P>;
-    allow_lds((const void*)k, smem);
     ktimer_begin(KC_TOEPLITZ_ROWS_FWD, stream);
-    hipLaunchKernelGGL(k, dim3(nbps * nslices), dim3(P::NT), smem, stream, x,
-                       wt, tw, n, H, rp, nbps);
+    auto launch = [&](auto k) {
+      allow_lds((const void*)k, smem);
+      hipLaunchKernelGGL(k, dim3(nbps * nslices), dim3(P::NT), smem, stream,
+                         x, wt, tw, n, H, rp, nbps, cp);
+    };
+    if (cm == 0) launch(ct_rows_fwd<P, 0>);
+    else if (cm == 1) launch(ct_rows_fwd<P, 1>);
+    else launch(ct_rows_fwd<P, 2>);
     TC_HIP(hipGetLastError());
-    ktimer_end(KC_TOEPLITZ_ROWS_FWD, stream, img_b + spec_b);
+    // g in, Wt out; committing: sol (and xp) in, dst out
+    ktimer_end(KC_TOEPLITZ_ROWS_FWD, stream,
+               img_b + spec_b + (cm ? (cm == 2 ? 3.0 : 2.0) * img_b : 0.0));
   }
   {
     const int nbps = (int)cdiv(H, P::LPW);
@@ -254,21 +311,21 @@ static void run_ct(const float* x, const float* b, float* out, float2* wt,
   }
   {
     const int nbps = (int)cdiv(npairs, P::LPW);
+    const int em = cm ? cm + 1 : mode;  // epilogue mode
     ktimer_begin(KC_TOEPLITZ_ROWS_INV, stream);
-    if (mode == 0) {
-      auto k = ct_rows_inv<P, 0>;
+    auto launch = [&](auto k) {
       allow_lds((const void*)k, smem);
       hipLaunchKernelGGL(k, dim3(nbps * nslices), dim3(P::NT), smem, stream,
-                         wt, b, out, tw, n, H, rp, nbps);
-    } else {
-      auto k = ct_rows_inv<P, 1>;
-      allow_lds((const void*)k, smem);
-      hipLaunchKernelGGL(k, dim3(nbps * nslices), dim3(P::NT), smem, stream,
-                         wt, b, out, tw, n, H, rp, nbps);
-    }
+                         wt, b, out, tw, n, H, rp, nbps, cp);
+    };
+    if (em == 0) launch(ct_rows_inv<P, 0>);
+    else if (em == 1) launch(ct_rows_inv<P, 1>);
+    else if (em == 2) launch(ct_rows_inv<P, 2>);
+    else launch(ct_rows_inv<P, 3>);
     TC_HIP(hipGetLastError());
-    ktimer_end(KC_TOEPLITZ_ROWS_INV, stream,
-               spec_b + img_b * (mode == 1 ? 2.0 : 1.0));
+    // Wt in, out written; plus b (mode 1) or Tsol (and Tx) when committing
+    const double extra = em == 1 ? 1.0 : (em == 2 ? 1.0 : (em == 3 ? 2.0 : 0.0));
+    ktimer_end(KC_TOEPLITZ_ROWS_INV, stream, spec_b + img_b * (1.0 + extra));
   }
 }
 
@@ -285,23 +342,24 @@ int toeplitz_ct_lpw(int M) {
 bool toeplitz_apply_ct(int M, const float* x, const float* b, float* out,
                        float2* wt, const float* spec, int n, int H, int rp,
                        int nslices, int mode, double* partials,
-                       hipStream_t stream, double img_b, double spec_b) {
+                       hipStream_t stream, double img_b, double spec_b,
+                       const Psf::StepCommit* sc) {
   switch (M) {
     case 750:
       run_ct<Plan750>(x, b, out, wt, spec, n, H, rp, nslices, mode, partials,
-                      stream, img_b, spec_b);
+                      stream, img_b, spec_b, sc);
       return true;
     case 1458:
       run_ct<Plan1458>(x, b, out, wt, spec, n, H, rp, nslices, mode, partials,
-                       stream, img_b, spec_b);
+                       stream, img_b, spec_b, sc);
       return true;
     case 2916:
       run_ct<Plan2916>(x, b, out, wt, spec, n, H, rp, nslices, mode, partials,
-                       stream, img_b, spec_b);
+                       stream, img_b, spec_b, sc);
       return true;
     case 5832:
       run_ct<Plan5832>(x, b, out, wt, spec, n, H, rp, nslices, mode, partials,
-                       stream, img_b, spec_b);
+                       stream, img_b, spec_b, sc);
       return true;
     default:
       return false;
