@@ -322,9 +322,9 @@ static void qggmrf_grid(int S, int R, int C, int* zchunk, unsigned* nblk) {
   const long ntiles = cdiv(C, QCOLS) * cdiv(R, QROWS * QWAVES);
   int best = S;
   double best_cost = 1e300;
-  for (int zc = S; zc >= 1; --zc) {
-    const long nz = cdiv(S, zc);
-    if (zc != S && cdiv(S, zc + 1) == nz) continue;  // same count, longer
+  for (int nz = 1; nz <= S; ++nz) {
+    const int zc = (int)cdiv(S, nz);  // shortest chunk giving nz chunks
+    if (cdiv(S, zc) != nz) continue;
     const long waves = ntiles * QWAVES * nz;
     const double rounds = (double)cdiv(waves, simds);
     const double cost = rounds * (zc + 0.7);

@@ -59,8 +59,7 @@ static void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w
 // ---------------------------------------------------------------------------
 // spreading (type 1): each workgroup owns a T x T tile of the fine grid and
 // gathers every ray segment whose kernel footprint meets it (rays are
-// enumerated analytically, so no sorting and no global atomics).  A wave
-// processes one point at a time, lane (a, b) owning footprint cell (a, b).
+// enumerated analytically, so no sorting and no global atomics).
 // ---------------------------------------------------------------------------
 
 constexpr int SP_T = 32;
@@ -236,24 +235,40 @@ __global__ void __launch_bounds__(256) spread_count_kernel(SpreadArgs a,
 }
 
 // One workgroup per (work item, slice group): accumulates the points
-// [f0, f1) of its tile in LDS (one point per lane) and writes the tile to the
-// fine grid (first chunk) or to a scratch slot (further chunks).
+// [f0, f1) of its tile in LDS and writes the tile to the fine grid (first
+// chunk) or to a scratch slot (further chunks).  Each wave walks its own
+// contiguous quarter of the points in batches of 64: the lanes first load
+// one point each (coordinates, slice values) in parallel, then the batch is
+// spread one point at a time with lane (a, b) on footprint cell (a, b), so
+// a point costs two kernel evaluations per lane and its LDS updates hit 64
+// distinct cells.  The accumulators are split
+// into real and imaginary planes with row stride SP_RS = 40 floats, which
+// puts the 8 x 8 footprint on all 32 banks (two lanes per bank).
+constexpr int SP_RS = 40;
+
 __global__ void __launch_bounds__(256)
 spread_kernel(SpreadArgs a, const int4* __restrict__ work, float2* scratch) {
-  __shared__ float2 acc[SP_SB][SP_T * SP_T];
+  __shared__ float accx[SP_SB][SP_T * SP_RS];
+  __shared__ float accy[SP_SB][SP_T * SP_RS];
   __shared__ int seg_pj[SP_LIST];
   __shared__ int seg_off[SP_LIST + 1];
   __shared__ int wsum[4];
   __shared__ int nseg;
   const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int qa = lane >> 3, qb = lane & 7;
   const int4 item = work[xcd_remap(blockIdx.x, gridDim.x)];
   const int s0 = blockIdx.y * SP_SB;
   const int ns = min(SP_SB, a.nsl - s0);
   const TileWindow w = tile_window(a, item.x);
-  for (int i = tid; i < SP_SB * SP_T * SP_T; i += 256)
-    (&acc[0][0])[i] = make_float2(0.f, 0.f);
+  for (int i = tid; i < SP_SB * SP_T * SP_RS; i += 256) {
+    (&accx[0][0])[i] = 0.f;
+    (&accy[0][0])[i] = 0.f;
+  }
   const double hw = 0.5 * NUFFT_W;
   const long sstride = (long)a.P * a.N;
+  const float zscale = 2.0f / NUFFT_W;
   int base = 0;
   for (int ky = -1; ky <= 1; ++ky) {
     for (int kx = -1; kx <= 1; ++kx) {
@@ -267,49 +282,76 @@ spread_kernel(SpreadArgs a, const int4* __restrict__ work, float2* scratch) {
                                            &nseg, &nsg);
       const int fa = max(item.y, base) - base;
       const int fb = min(item.z, base + total) - base;
-      for (int f = fa + tid; f < fb; f += 256) {
-        int lo = 0, hi = nsg;  // seg_off[lo] <= f < seg_off[hi]
+      const int per = (fb - fa + 3) >> 2;
+      int f = fa + wave * per;
+      const int fend = min(fb, f + per);
+      if (f < fend) {
+        // segment holding f: seg_off[k] <= f < seg_off[k + 1]
+        int lo = 0, hi = nsg;
         while (hi - lo > 1) {
           const int mid = (lo + hi) >> 1;
-          if (seg_off[mid] <= f) lo = mid; else hi = mid;
+          if (__builtin_amdgcn_readfirstlane(seg_off[mid]) <= f) lo = mid;
+          else hi = mid;
         }
-        const int pj = seg_pj[lo];
-        const int p = pj & 2047;
-        const int j = (pj >> 11) + (f - seg_off[lo]);
-        const double t = (double)(j - a.cidx);
-        const double u = U0 + t * a.ax[p];
-        const double v = V0 + t * a.ay[p];
-        const int l0x = (int)ceil(u - hw);
-        const int l0y = (int)ceil(v - hw);
-        if (l0x + NUFFT_W <= w.X0 || l0x >= w.X1 || l0y + NUFFT_W <= w.Y0 ||
-            l0y >= w.Y1)
-          continue;
-        float wx[NUFFT_W], wy[NUFFT_W];
+        int k0 = lo;
+        for (int fb0 = f; fb0 < fend; fb0 += 64) {
+          const int nb = min(64, fend - fb0);
+          // lanes load the batch's points in parallel: cell origin, offsets
+          // of the first footprint cell, and the slices' values
+          int k = k0, l0x = 0, l0y = 0;
+          float a0 = 0.f, b0 = 0.f;
+          float2 cv[SP_SB];
 #pragma unroll
-        for (int q = 0; q < NUFFT_W; ++q) {
-          wx[q] = es_kernel((float)((l0x + q - u) * (2.0 / NUFFT_W)), a.beta);
-          wy[q] = es_kernel((float)((l0y + q - v) * (2.0 / NUFFT_W)), a.beta);
-        }
-        float2 cv[SP_SB];
-        const float2* cp = a.c + (long)p * a.N + j;
+          for (int q = 0; q < SP_SB; ++q) cv[q] = make_float2(0.f, 0.f);
+          if (lane < nb) {
+            const int fl = fb0 + lane;
+            while (fl >= seg_off[k + 1]) ++k;
+            const int pj = seg_pj[k];
+            const int p = pj & 2047;
+            const int j = (pj >> 11) + (fl - seg_off[k]);
+            const double t = (double)(j - a.cidx);
+            const double u = U0 + t * a.ax[p];
+            const double v = V0 + t * a.ay[p];
+            const double cxu = ceil(u - hw), cyv = ceil(v - hw);
+            l0x = (int)cxu;
+            l0y = (int)cyv;
+            a0 = (float)(cxu - u);
+            b0 = (float)(cyv - v);
+            const float2* cp = a.c + (long)s0 * sstride + (long)p * a.N + j;
 #pragma unroll
-        for (int s = 0; s < SP_SB; ++s)
-          cv[s] = s < ns ? cp[(long)(s0 + s) * sstride] : make_float2(0.f, 0.f);
+            for (int q = 0; q < SP_SB; ++q)
+              if (q < ns) cv[q] = cp[(long)q * sstride];
+          }
+          k0 = __builtin_amdgcn_readlane(k, nb - 1);
+          // then spread them one at a time, lane (a, b) on footprint cell
+          // (a, b); the per-point values are broadcast from their lane
+          for (int i = 0; i < nb; ++i) {
+            const int cx = __builtin_amdgcn_readlane(l0x, i) + qb;
+            const int cy = __builtin_amdgcn_readlane(l0y, i) + qa;
+            if (cx >= w.X0 && cx < w.X1 && cy >= w.Y0 && cy < w.Y1) {
+              const float dx =
+                  __builtin_bit_cast(float, __builtin_amdgcn_readlane(
+                                                __builtin_bit_cast(int, a0), i)) +
+                  (float)qb;
+              const float dy =
+                  __builtin_bit_cast(float, __builtin_amdgcn_readlane(
+                                                __builtin_bit_cast(int, b0), i)) +
+                  (float)qa;
+              const float wgt = es_kernel(dx * zscale, a.beta) *
+                                es_kernel(dy * zscale, a.beta);
+              const int cell = (cy - w.Y0) * SP_RS + (cx - w.X0);
 #pragma unroll
-        for (int qa = 0; qa < NUFFT_W; ++qa) {
-          const int cy = l0y + qa;
-          if (cy < w.Y0 || cy >= w.Y1) continue;
-#pragma unroll
-          for (int qb = 0; qb < NUFFT_W; ++qb) {
-            const int cx = l0x + qb;
-            if (cx < w.X0 || cx >= w.X1) continue;
-            const float wgt = wy[qa] * wx[qb];
-            const int cell = (cy - w.Y0) * SP_T + (cx - w.X0);
-#pragma unroll
-            for (int s = 0; s < SP_SB; ++s) {
-              if (s < ns) {
-                atomicAdd(&acc[s][cell].x, cv[s].x * wgt);
-                atomicAdd(&acc[s][cell].y, cv[s].y * wgt);
+              for (int q = 0; q < SP_SB; ++q) {
+                if (q < ns) {
+                  const float vx = __builtin_bit_cast(
+                      float, __builtin_amdgcn_readlane(
+                                 __builtin_bit_cast(int, cv[q].x), i));
+                  const float vy = __builtin_bit_cast(
+                      float, __builtin_amdgcn_readlane(
+                                 __builtin_bit_cast(int, cv[q].y), i));
+                  atomicAdd(&accx[q][cell], vx * wgt);
+                  atomicAdd(&accy[q][cell], vy * wgt);
+                }
               }
             }
           }
@@ -319,19 +361,20 @@ spread_kernel(SpreadArgs a, const int4* __restrict__ work, float2* scratch) {
     }
   }
   __syncthreads();
-  if (item.w < 0) {
-    for (int i = tid; i < ns * SP_T * SP_T; i += 256) {
-      const int s = i / (SP_T * SP_T);
-      const int cell = i - s * SP_T * SP_T;
-      const int ly = cell / SP_T, lx = cell - ly * SP_T;
-      if (w.Y0 + ly < w.Y1 && w.X0 + lx < w.X1)
-        a.fine[((long)(s0 + s) * a.nf + w.Y0 + ly) * a.nf + w.X0 + lx] =
-            acc[s][cell];
+  float2* dst = item.w < 0 ? nullptr
+                           : scratch + ((long)item.w * gridDim.y + blockIdx.y) *
+                                           (SP_SB * SP_T * SP_T);
+  for (int i = tid; i < ns * SP_T * SP_T; i += 256) {
+    const int s = i / (SP_T * SP_T);
+    const int c = i - s * SP_T * SP_T;
+    const int ly = c / SP_T, lx = c - ly * SP_T;
+    const float2 val =
+        make_float2(accx[s][ly * SP_RS + lx], accy[s][ly * SP_RS + lx]);
+    if (dst) {
+      dst[i] = val;
+    } else if (w.Y0 + ly < w.Y1 && w.X0 + lx < w.X1) {
+      a.fine[((long)(s0 + s) * a.nf + w.Y0 + ly) * a.nf + w.X0 + lx] = val;
     }
-  } else {
-    float2* dst = scratch + ((long)item.w * gridDim.y + blockIdx.y) *
-                                (SP_SB * SP_T * SP_T);
-    for (int i = tid; i < ns * SP_T * SP_T; i += 256) dst[i] = (&acc[0][0])[i];
   }
 }
 

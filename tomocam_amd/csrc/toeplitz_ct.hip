@@ -140,8 +140,12 @@ ct_cols(float2* __restrict__ wt, const float* __restrict__ spec,
   const int line = threadIdx.x / T;
   const int t = threadIdx.x - line * T;
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int s = b / nbps;
-  const int k1 = (b - s * nbps) * LPW + line;
+  // slice-minor order: the workgroups that read spectrum rows k1 for every
+  // slice run back to back on one XCD, so each row is fetched once into L2
+  const int nslices = gridDim.x / nbps;
+  const int kg = b / nslices;
+  const int s = b - kg * nslices;
+  const int k1 = kg * LPW + line;
   const bool active = k1 < H;
   float2* col = wt + ((long)s * H + (active ? k1 : 0)) * rp;
   float2 v[E];
