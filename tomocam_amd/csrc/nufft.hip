@@ -17,9 +17,14 @@ namespace tc {
 // kernel function and host-side correction factors
 // ---------------------------------------------------------------------------
 
+// exp(beta (sqrt(1 - z^2) - 1)) on |z| <= 1, else 0.  Hardware sqrt and
+// exp2 (about 1 ulp each): the kernel's own approximation error is ~1e-7.
 __device__ __forceinline__ float es_kernel(float z, float beta) {
   const float s = 1.0f - z * z;
-  return s >= 0.f ? __expf(beta * (sqrtf(s) - 1.0f)) : 0.f;
+  const float e = __builtin_amdgcn_exp2f(
+      beta * 1.44269504088896341f *
+      (__builtin_amdgcn_sqrtf(fmaxf(s, 0.f)) - 1.0f));
+  return s >= 0.f ? e : 0.f;
 }
 
 // Gauss-Legendre nodes/weights on [-1, 1] (Newton iteration on P_n).
@@ -236,24 +241,33 @@ __global__ void __launch_bounds__(256) spread_count_kernel(SpreadArgs a,
 
 // One workgroup per (work item, slice group): accumulates the points
 // [f0, f1) of its tile in LDS and writes the tile to the fine grid (first
-// chunk) or to a scratch slot (further chunks).  Each wave walks its own
-// contiguous quarter of the points in batches of 64: the lanes first load
-// one point each (coordinates, slice values) in parallel, then the batch is
-// spread one point at a time with lane (a, b) on footprint cell (a, b), so
-// a point costs two kernel evaluations per lane and its LDS updates hit 64
-// distinct cells.  The accumulators are split
-// into real and imaginary planes with row stride SP_RS = 40 floats, which
-// puts the 8 x 8 footprint on all 32 banks (two lanes per bank).
-constexpr int SP_RS = 40;
+// chunk) or to a scratch slot (further chunks).  Points are taken in batches
+// of SP_BATCH: each thread first prepares one point (cell origin, offsets of
+// the first footprint cell, slice values, the 8-row bands of the tile its
+// footprint meets) in LDS; then wave w adds the batch's points that meet
+// band w (rows [8w, 8w + 8) of the tile, which it owns) with lane (a, b) on
+// footprint cell (a, b).  Owned rows make the read-add-write race-free
+// without atomics (LDS float atomics are ~30x slower than a plain
+// read-add-write here), and each cell sums its points in list order, so the
+// result is deterministic.  The segment list lives in dynamic LDS sized to
+// the number of angles.
+constexpr int SP_RS = 36;  // accumulator row stride (float2)
+constexpr int SP_BATCH = 128;
+static_assert(SP_SB == 4, "slice values are held as two float4 per point");
 
 __global__ void __launch_bounds__(256)
 spread_kernel(SpreadArgs a, const int4* __restrict__ work, float2* scratch) {
-  __shared__ float accx[SP_SB][SP_T * SP_RS];
-  __shared__ float accy[SP_SB][SP_T * SP_RS];
-  __shared__ int seg_pj[SP_LIST];
-  __shared__ int seg_off[SP_LIST + 1];
+  __shared__ float2 acc[SP_SB][SP_T * SP_RS];
+  __shared__ int4 pt_geo[SP_BATCH];      // l0x, l0y, bits(a0), bits(b0)
+  __shared__ float4 pt_val[SP_BATCH][2];  // slices (0, 1), (2, 3)
+  __shared__ int pt_band[SP_BATCH];
+  __shared__ short wlist[4][SP_BATCH];
   __shared__ int wsum[4];
   __shared__ int nseg;
+  extern __shared__ int seg_dyn[];  // seg_pj[P], seg_off[P + 1]
+  int* seg_pj = seg_dyn;
+  int* seg_off = seg_dyn + a.P;
+  static_assert(SP_T == 32, "four 8-row bands per tile");
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -262,10 +276,10 @@ spread_kernel(SpreadArgs a, const int4* __restrict__ work, float2* scratch) {
   const int s0 = blockIdx.y * SP_SB;
   const int ns = min(SP_SB, a.nsl - s0);
   const TileWindow w = tile_window(a, item.x);
-  for (int i = tid; i < SP_SB * SP_T * SP_RS; i += 256) {
-    (&accx[0][0])[i] = 0.f;
-    (&accy[0][0])[i] = 0.f;
-  }
+  const int band0 = w.Y0 + 8 * wave;  // rows this wave owns
+  const int band1 = min(band0 + 8, w.Y1);
+  for (int i = tid; i < SP_SB * SP_T * SP_RS; i += 256)
+    (&acc[0][0])[i] = make_float2(0.f, 0.f);
   const double hw = 0.5 * NUFFT_W;
   const long sstride = (long)a.P * a.N;
   const float zscale = 2.0f / NUFFT_W;
@@ -282,80 +296,111 @@ spread_kernel(SpreadArgs a, const int4* __restrict__ work, float2* scratch) {
                                            &nseg, &nsg);
       const int fa = max(item.y, base) - base;
       const int fb = min(item.z, base + total) - base;
-      const int per = (fb - fa + 3) >> 2;
-      int f = fa + wave * per;
-      const int fend = min(fb, f + per);
-      if (f < fend) {
-        // segment holding f: seg_off[k] <= f < seg_off[k + 1]
-        int lo = 0, hi = nsg;
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (__builtin_amdgcn_readfirstlane(seg_off[mid]) <= f) lo = mid;
-          else hi = mid;
-        }
-        int k0 = lo;
-        for (int fb0 = f; fb0 < fend; fb0 += 64) {
-          const int nb = min(64, fend - fb0);
-          // lanes load the batch's points in parallel: cell origin, offsets
-          // of the first footprint cell, and the slices' values
-          int k = k0, l0x = 0, l0y = 0;
-          float a0 = 0.f, b0 = 0.f;
+      for (int fb0 = fa; fb0 < fb; fb0 += SP_BATCH) {
+        const int nb = min(SP_BATCH, fb - fb0);
+        // prepare: thread t -> point fb0 + t
+        if (tid < nb) {
+          const int f = fb0 + tid;
+          int lo = 0, hi = nsg;  // seg_off[lo] <= f < seg_off[hi]
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (seg_off[mid] <= f) lo = mid;
+            else hi = mid;
+          }
+          const int pj = seg_pj[lo];
+          const int p = pj & 2047;
+          const int j = (pj >> 11) + (f - seg_off[lo]);
+          const double t = (double)(j - a.cidx);
+          const double u = U0 + t * a.ax[p];
+          const double v = V0 + t * a.ay[p];
+          const double cxu = ceil(u - hw), cyv = ceil(v - hw);
+          const int l0x = (int)cxu, l0y = (int)cyv;
+          pt_geo[tid] = make_int4(l0x, l0y,
+                                  __builtin_bit_cast(int, (float)(cxu - u)),
+                                  __builtin_bit_cast(int, (float)(cyv - v)));
+          const float2* cp = a.c + (long)s0 * sstride + (long)p * a.N + j;
           float2 cv[SP_SB];
 #pragma unroll
-          for (int q = 0; q < SP_SB; ++q) cv[q] = make_float2(0.f, 0.f);
-          if (lane < nb) {
-            const int fl = fb0 + lane;
-            while (fl >= seg_off[k + 1]) ++k;
-            const int pj = seg_pj[k];
-            const int p = pj & 2047;
-            const int j = (pj >> 11) + (fl - seg_off[k]);
-            const double t = (double)(j - a.cidx);
-            const double u = U0 + t * a.ax[p];
-            const double v = V0 + t * a.ay[p];
-            const double cxu = ceil(u - hw), cyv = ceil(v - hw);
-            l0x = (int)cxu;
-            l0y = (int)cyv;
-            a0 = (float)(cxu - u);
-            b0 = (float)(cyv - v);
-            const float2* cp = a.c + (long)s0 * sstride + (long)p * a.N + j;
+          for (int q = 0; q < SP_SB; ++q)
+            cv[q] = q < ns ? cp[(long)q * sstride] : make_float2(0.f, 0.f);
+          pt_val[tid][0] = make_float4(cv[0].x, cv[0].y, cv[1].x, cv[1].y);
+          pt_val[tid][1] = make_float4(cv[2].x, cv[2].y, cv[3].x, cv[3].y);
+          int bands = 0;
+          if (l0x < w.X1 && l0x + NUFFT_W > w.X0) {
 #pragma unroll
-            for (int q = 0; q < SP_SB; ++q)
-              if (q < ns) cv[q] = cp[(long)q * sstride];
-          }
-          k0 = __builtin_amdgcn_readlane(k, nb - 1);
-          // then spread them one at a time, lane (a, b) on footprint cell
-          // (a, b); the per-point values are broadcast from their lane
-          for (int i = 0; i < nb; ++i) {
-            const int cx = __builtin_amdgcn_readlane(l0x, i) + qb;
-            const int cy = __builtin_amdgcn_readlane(l0y, i) + qa;
-            if (cx >= w.X0 && cx < w.X1 && cy >= w.Y0 && cy < w.Y1) {
-              const float dx =
-                  __builtin_bit_cast(float, __builtin_amdgcn_readlane(
-                                                __builtin_bit_cast(int, a0), i)) +
-                  (float)qb;
-              const float dy =
-                  __builtin_bit_cast(float, __builtin_amdgcn_readlane(
-                                                __builtin_bit_cast(int, b0), i)) +
-                  (float)qa;
-              const float wgt = es_kernel(dx * zscale, a.beta) *
-                                es_kernel(dy * zscale, a.beta);
-              const int cell = (cy - w.Y0) * SP_RS + (cx - w.X0);
-#pragma unroll
-              for (int q = 0; q < SP_SB; ++q) {
-                if (q < ns) {
-                  const float vx = __builtin_bit_cast(
-                      float, __builtin_amdgcn_readlane(
-                                 __builtin_bit_cast(int, cv[q].x), i));
-                  const float vy = __builtin_bit_cast(
-                      float, __builtin_amdgcn_readlane(
-                                 __builtin_bit_cast(int, cv[q].y), i));
-                  atomicAdd(&accx[q][cell], vx * wgt);
-                  atomicAdd(&accy[q][cell], vy * wgt);
-                }
-              }
+            for (int b = 0; b < 4; ++b) {
+              const int r0 = w.Y0 + 8 * b, r1 = min(r0 + 8, w.Y1);
+              if (l0y < r1 && l0y + NUFFT_W > r0) bands |= 1 << b;
             }
           }
+          pt_band[tid] = bands;
         }
+        __syncthreads();
+        // this wave's list of points meeting its band, kept in registers
+        int cnt = 0;
+        int lst[SP_BATCH / 64];
+#pragma unroll
+        for (int c = 0; c < SP_BATCH / 64; ++c) {
+          const int idx = c * 64 + lane;
+          const bool hit = idx < nb && ((pt_band[idx] >> wave) & 1);
+          const unsigned long long bal = __ballot(hit);
+          const int pos =
+              cnt + __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                              __builtin_amdgcn_mbcnt_lo(
+                                                  (unsigned)bal, 0u));
+          if (hit) wlist[wave][pos] = (short)idx;
+          cnt += __popcll(bal);
+        }
+#pragma unroll
+        for (int c = 0; c < SP_BATCH / 64; ++c) lst[c] = wlist[wave][c * 64 + lane];
+        // software-pipelined: point k + 1's data is read from LDS while
+        // point k's cells are updated
+        auto entry = [&](int k) {
+          return k < 64 ? __builtin_amdgcn_readlane(lst[0], k)
+                        : __builtin_amdgcn_readlane(lst[1], k - 64);
+        };
+        static_assert(SP_BATCH == 128, "two list registers");
+        int4 g = make_int4(0, 0, 0, 0);
+        float4 v01 = make_float4(0.f, 0.f, 0.f, 0.f), v23 = v01;
+        if (cnt > 0) {
+          const int i = entry(0);
+          g = pt_geo[i];
+          v01 = pt_val[i][0];
+          v23 = pt_val[i][1];
+        }
+        for (int k = 0; k < cnt; ++k) {
+          const int in = entry(min(k + 1, cnt - 1));
+          const int4 gn = pt_geo[in];
+          const float4 v01n = pt_val[in][0];
+          const float4 v23n = pt_val[in][1];
+          const int cx = g.x + qb;
+          const int cy = g.y + qa;
+          if (cy >= band0 && cy < band1 && cx >= w.X0 && cx < w.X1) {
+            const float dx = __builtin_bit_cast(float, g.z) + (float)qb;
+            const float dy = __builtin_bit_cast(float, g.w) + (float)qa;
+            const float wgt = es_kernel(dx * zscale, a.beta) *
+                              es_kernel(dy * zscale, a.beta);
+            const int cell = (cy - w.Y0) * SP_RS + (cx - w.X0);
+            float2 a0 = acc[0][cell], a1 = acc[1][cell];
+            float2 a2 = acc[2][cell], a3 = acc[3][cell];
+            a0.x = fmaf(v01.x, wgt, a0.x);
+            a0.y = fmaf(v01.y, wgt, a0.y);
+            a1.x = fmaf(v01.z, wgt, a1.x);
+            a1.y = fmaf(v01.w, wgt, a1.y);
+            a2.x = fmaf(v23.x, wgt, a2.x);
+            a2.y = fmaf(v23.y, wgt, a2.y);
+            a3.x = fmaf(v23.z, wgt, a3.x);
+            a3.y = fmaf(v23.w, wgt, a3.y);
+            acc[0][cell] = a0;
+            acc[1][cell] = a1;
+            acc[2][cell] = a2;
+            acc[3][cell] = a3;
+          }
+          g = gn;
+          v01 = v01n;
+          v23 = v23n;
+        }
+        __syncthreads();  // the batch buffers are rewritten next
       }
       base += total;
     }
@@ -368,8 +413,7 @@ spread_kernel(SpreadArgs a, const int4* __restrict__ work, float2* scratch) {
     const int s = i / (SP_T * SP_T);
     const int c = i - s * SP_T * SP_T;
     const int ly = c / SP_T, lx = c - ly * SP_T;
-    const float2 val =
-        make_float2(accx[s][ly * SP_RS + lx], accy[s][ly * SP_RS + lx]);
+    const float2 val = acc[s][ly * SP_RS + lx];
     if (dst) {
       dst[i] = val;
     } else if (w.Y0 + ly < w.Y1 && w.X0 + lx < w.X1) {
@@ -643,8 +687,10 @@ void PolarNufft::type1_real(const float2* c, float* out, int S, float scale,
     a.umax = umax;
     a.beta = beta_;
     const unsigned ng = cdiv(nb, SP_SB);
-    hipLaunchKernelGGL(spread_kernel, dim3((unsigned)nwork_, ng), dim3(256), 0,
-                       stream, a, work_.as<int4>(), scratch.as<float2>());
+    const size_t seg_lds = sizeof(int) * (2 * (size_t)P_ + 1);
+    hipLaunchKernelGGL(spread_kernel, dim3((unsigned)nwork_, ng), dim3(256),
+                       seg_lds, stream, a, work_.as<int4>(),
+                       scratch.as<float2>());
     TC_HIP(hipGetLastError());
     if (nfix_) {
       hipLaunchKernelGGL(spread_fixup_kernel, dim3((unsigned)nfix_, ng),

@@ -9,6 +9,9 @@
 
 namespace tc {
 
+// lines per workgroup of the compile-time plan for FFT length M (0: none)
+int toeplitz_ct_lpw(int M);
+
 // ---------------------------------------------------------------------------
 // PSF spectrum
 // ---------------------------------------------------------------------------
@@ -44,7 +47,7 @@ Psf::Psf(const float* d_psf, int n, hipStream_t stream) : n_(n) {
   TC_REQUIRE(n >= 2, "PSF: image width must be >= 2");
   M_ = fft_next_smooth(2 * n - 1, true);
   H_ = M_ / 2 + 1;
-  rp_ = n + (n & 1);
+  rp_ = n + (n & 1);  // rows of the workspace: whole pairs
   const size_t n1 = 2 * (size_t)n - 1;
   arr_.alloc(sizeof(float) * n1 * n1);
   TC_HIP(hipMemcpyAsync(arr_.as<void>(), d_psf, sizeof(float) * n1 * n1,
@@ -237,7 +240,6 @@ void reduce_partials(const double* partials, long n, double* out,
   TC_HIP(hipGetLastError());
 }
 
-int toeplitz_ct_lpw(int M);
 bool toeplitz_apply_ct(int M, const float* x, const float* b, float* out,
                        float2* wt, const float* spec, int n, int H, int rp,
                        int nslices, int mode, double* partials,
