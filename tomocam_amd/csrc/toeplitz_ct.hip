@@ -65,14 +65,6 @@ struct CommitPtrs {
 //   (+ dst = y - s x when committing a step: x is the gradient there)
 // ---------------------------------------------------------------------------
 
-template <int CM>
-__device__ __forceinline__ void commit_elem(const CommitPtrs& c, long o,
-                                            float gv) {
-  float y = c.sol[o];
-  if constexpr (CM == 2) y = fmaf(c.beta, y - c.xp[o], y);
-  c.dst[o] = fmaf(-c.s, gv, y);
-}
-
 template <class P, int CM>
 __global__ void __launch_bounds__(P::NT)
 ct_rows_fwd(const float* __restrict__ x, float2* __restrict__ wt,
@@ -91,24 +83,51 @@ ct_rows_fwd(const float* __restrict__ x, float2* __restrict__ wt,
   const float* xs = x + (long)s * n * n;
   const float* rowa = xs + (long)ra * n;
   const bool hasa = ra < n, hasb = ra + 1 < n;
+  // All loads first, then the commit's stores: interleaved, the compiler
+  // cannot prove the stores miss later loads (dst may alias sol) and would
+  // wait out each load's latency in turn.
   float2 v[E];
+  float2 y[CM != 0 ? E : 1];
+  const long oa = (long)s * n * n + (long)ra * n;
 #pragma unroll
   for (int i = 0; i < E / R1; ++i)
 #pragma unroll
     for (int r = 0; r < R1; ++r) {
       const int c = t + i * T + r * (L / R1);
+      const int e = i * R1 + r;
       float re = 0.f, im = 0.f;
       if (c < n) {
         if (hasa) re = rowa[c];
         if (hasb) im = rowa[n + c];
         if constexpr (CM != 0) {
-          const long o = (long)s * n * n + (long)ra * n + c;
-          if (hasa) commit_elem<CM>(cp, o, re);
-          if (hasb) commit_elem<CM>(cp, o + n, im);
+          float ya = 0.f, yb = 0.f;
+          if (hasa) {
+            ya = cp.sol[oa + c];
+            if constexpr (CM == 2) ya = fmaf(cp.beta, ya - cp.xp[oa + c], ya);
+          }
+          if (hasb) {
+            yb = cp.sol[oa + n + c];
+            if constexpr (CM == 2)
+              yb = fmaf(cp.beta, yb - cp.xp[oa + n + c], yb);
+          }
+          y[e] = make_float2(ya, yb);
         }
       }
-      v[i * R1 + r] = make_float2(re, im);
+      v[e] = make_float2(re, im);
     }
+  if constexpr (CM != 0) {
+#pragma unroll
+    for (int i = 0; i < E / R1; ++i)
+#pragma unroll
+      for (int r = 0; r < R1; ++r) {
+        const int c = t + i * T + r * (L / R1);
+        const int e = i * R1 + r;
+        if (c < n) {
+          if (hasa) cp.dst[oa + c] = fmaf(-cp.s, v[e].x, y[e].x);
+          if (hasb) cp.dst[oa + n + c] = fmaf(-cp.s, v[e].y, y[e].y);
+        }
+      }
+  }
   float2* ln = sm + line * L;
   F::run(v, ln, t, true, tw);
 #pragma unroll
@@ -209,19 +228,6 @@ ct_cols(float2* __restrict__ wt, const float* __restrict__ spec,
 //   2 / 3: out = Ty - s T x with Ty = Tsol (2) or Tsol + beta (Tsol - Tx) (3)
 // ---------------------------------------------------------------------------
 
-template <int MODE>
-__device__ __forceinline__ float rows_inv_epilogue(float q, const float* bvec,
-                                                   const CommitPtrs& c,
-                                                   long o) {
-  if constexpr (MODE == 0) return q;
-  else if constexpr (MODE == 1) return q - bvec[o];
-  else {
-    float ty = c.Tsol[o];
-    if constexpr (MODE == 3) ty = fmaf(c.beta, ty - c.Tx[o], ty);
-    return fmaf(-c.s, q, ty);
-  }
-}
-
 template <class P, int MODE>
 __global__ void __launch_bounds__(P::NT)
 ct_rows_inv(const float2* __restrict__ wt, const float* bvec, float* out,
@@ -260,22 +266,55 @@ ct_rows_inv(const float2* __restrict__ wt, const float* bvec, float* out,
   __syncthreads();
   I::run(v, ln, t, true, tw);
   const int ra = r0 + 2 * line;
-  const long so = (long)s * n * n;
+  const long oa = (long)s * n * n + (long)ra * n;
+  const bool hasa = ra < n, hasb = ra + 1 < n;
+  // epilogue operands (b, or T sol and T x) all loaded before any store,
+  // which the compiler could not otherwise move past (out may alias Tsol)
+  float2 e0[MODE != 0 ? E : 1], e1[MODE == 3 ? E : 1];
+  if constexpr (MODE != 0) {
+    const float* src0 = MODE == 1 ? bvec : cp.Tsol;
+#pragma unroll
+    for (int i = 0; i < E / RS; ++i)
+#pragma unroll
+      for (int r = 0; r < RS; ++r) {
+        const int c = t + i * T + r * (L / RS);
+        const int e = i * RS + r;
+        float2 p0 = make_float2(0.f, 0.f), p1 = p0;
+        if (c < n) {
+          if (hasa) p0.x = src0[oa + c];
+          if (hasb) p0.y = src0[oa + n + c];
+          if constexpr (MODE == 3) {
+            if (hasa) p1.x = cp.Tx[oa + c];
+            if (hasb) p1.y = cp.Tx[oa + n + c];
+          }
+        }
+        e0[e] = p0;
+        if constexpr (MODE == 3) e1[e] = p1;
+      }
+  }
 #pragma unroll
   for (int i = 0; i < E / RS; ++i)
 #pragma unroll
     for (int r = 0; r < RS; ++r) {
       const int c = t + i * T + r * (L / RS);
+      const int e = i * RS + r;
       if (c < n) {
-        const float2 q = v[i * RS + r];
-        if (ra < n) {
-          const long o = so + (long)ra * n + c;
-          out[o] = rows_inv_epilogue<MODE>(q.x, bvec, cp, o);
+        const float2 q = v[e];
+        float2 o = q;
+        if constexpr (MODE == 1) {
+          o.x = q.x - e0[e].x;
+          o.y = q.y - e0[e].y;
+        } else if constexpr (MODE >= 2) {
+          float2 ty = e0[e];
+          if constexpr (MODE == 3) {
+            ty.x = fmaf(cp.beta, ty.x - e1[e].x, ty.x);
+            ty.y = fmaf(cp.beta, ty.y - e1[e].y, ty.y);
+          }
+          o.x = fmaf(-cp.s, q.x, ty.x);
+          o.y = fmaf(-cp.s, q.y, ty.y);
         }
-        if (ra + 1 < n) {
-          const long o = so + (long)(ra + 1) * n + c;
-          out[o] = rows_inv_epilogue<MODE>(q.y, bvec, cp, o);
-        }
+        if (hasa) out[oa + c] = o.x;
+        if (hasb) out[oa + n + c] = o.y;
       }
     }
 }
