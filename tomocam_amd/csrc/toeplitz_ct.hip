@@ -242,19 +242,34 @@ ct_rows_inv(const float2* __restrict__ wt, const float* bvec, float* out,
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const int s = b / nbps;
   const int r0 = (b - s * nbps) * 2 * LPW;
-  for (int q = threadIdx.x; q < LPW * H; q += P::NT) {
+  // every Wt load of the thread in flight at once, then the LDS writes
+  constexpr int HC = L / 2 + 1;
+  constexpr int NQ = (LPW * HC + P::NT - 1) / P::NT;
+  float4 w4[NQ];
+#pragma unroll
+  for (int j = 0; j < NQ; ++j) {
+    const int q = threadIdx.x + j * P::NT;
     const int k = q / LPW;
-    const int l = q - k * LPW;
-    const int rr = r0 + 2 * l;
-    float4 w4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (rr < rp)
-      w4 = *reinterpret_cast<const float4*>(wt + wt_at(s, k, rr, H, rp));
-    if (k == 0 || 2 * k == L) {
-      w4.y = 0.f;
-      w4.w = 0.f;
+    const int rr = r0 + 2 * (q - k * LPW);
+    w4[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (q < LPW * H && rr < rp)
+      w4[j] = *reinterpret_cast<const float4*>(wt + wt_at(s, k, rr, H, rp));
+  }
+#pragma unroll
+  for (int j = 0; j < NQ; ++j) {
+    const int q = threadIdx.x + j * P::NT;
+    if (q < LPW * H) {
+      const int k = q / LPW;
+      const int l = q - k * LPW;
+      float4 w = w4[j];
+      if (k == 0 || 2 * k == L) {
+        w.y = 0.f;
+        w.w = 0.f;
+      }
+      sm[l * L + k] = make_float2(w.x - w.w, w.y + w.z);
+      if (k > 0 && L - k >= H)
+        sm[l * L + L - k] = make_float2(w.x + w.w, w.z - w.y);
     }
-    sm[l * L + k] = make_float2(w4.x - w4.w, w4.y + w4.z);
-    if (k > 0 && L - k >= H) sm[l * L + L - k] = make_float2(w4.x + w4.w, w4.z - w4.y);
   }
   __syncthreads();
   float2* ln = sm + line * L;
