@@ -44,6 +44,20 @@ WORKLOADS = {
 
 PEAK_HBM_GBS = 8000.0
 
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+# counter runs (calibrated on a copy kernel) for the default workload
+TRAFFIC_FILE = os.path.join(ROOT, "profiles",
+                            "mbir_fp32_32x1024_hbm_traffic.json")
+
+
+def measured_traffic(workload, kernel_class):
+    if workload != "plain-fp32-32x1024" or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        t = json.load(f)
+    c = t["classes"].get(kernel_class)
+    return None if c is None else c["traffic_bytes"]
+
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -87,8 +101,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="plain-fp32-32x1024",
                     choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-sample-slices", type=int, default=2)
-    ap.add_argument("--cpu-sample-iters", type=int, default=3)
+    ap.add_argument("--cpu-sample-slices", type=int, default=8)
+    ap.add_argument("--cpu-sample-iters", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ktimer", action="store_true",
                     help="do not bracket kernels with HIP events")
@@ -167,7 +181,8 @@ def main():
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": kd["gbs"] / PEAK_HBM_GBS,
-            "traffic": None,
+            # HBM bytes per launch (rocprofv3 counters), or None
+            "traffic": measured_traffic(args.workload, dom),
             "bytes_per_launch": kd["bytes_per_launch"],
             "avg_launch_ms": kd["avg_ms"],
         }
