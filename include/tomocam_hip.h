@@ -157,6 +157,20 @@ int tc_add_total_var(const float* d_x, float* d_g, int nslices, int nrows,
  * the value it would have added in *h_value (may be NULL). */
 int tc_add_tv_hessian(float* h_g, long n, float sigma, float* h_value);
 
+/* Differentiable form (no counterpart in the reference; it serves the
+ * torch.autograd projector/prior of tomocam_amd.autograd).  Free-boundary
+ * qGGMRF prior R(x) = sum over unordered in-bounds neighbour pairs
+ * {v, v+d} of w_d rho(x_v - x_{v+d}), rho(d) = |d/s|^2 / (c + |d/s|^(2-p)),
+ * with the reference's weights and constants:
+ *   tc_qggmrf_grad_free:  d_g += dR/dx (exact gradient of that sum; in the
+ *                         interior it equals tc_add_total_var2);
+ *   tc_qggmrf_value_free: *h_value = R(x), accumulated in FP64.
+ * Synchronises `stream` (value only). */
+int tc_qggmrf_grad_free(const float* d_x, float* d_g, int nslices, int nrows,
+                        int ncols, float sigma, float p, void* stream);
+int tc_qggmrf_value_free(const float* d_x, int nslices, int nrows, int ncols,
+                         float sigma, float p, double* h_value, void* stream);
+
 /* ---------------------------------------------------------------------- */
 /* MBIR iteration                                                           */
 /* ---------------------------------------------------------------------- */

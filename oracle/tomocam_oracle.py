@@ -547,6 +547,31 @@ def qggmrf_value(x: np.ndarray, sigma: float, p: float) -> float:
     return 0.5 * tot
 
 
+def qggmrf_grad_free(x: np.ndarray, sigma: float, p: float) -> np.ndarray:
+    """Exact gradient of :func:`qggmrf_value` (free boundary): for each voxel,
+    sum over the neighbours inside the volume of w_d rho'(x_v - x_{v+d}).
+    In the interior it equals :func:`qggmrf_grad`; at the faces the pairs that
+    leave the volume are dropped instead of clamped.  Used by the
+    differentiable form (the reference has no such function)."""
+    x = np.asarray(x, dtype=np.float64)
+    if x.ndim == 2:
+        x = x[None]
+    S, R, C = x.shape
+    xp = np.pad(x, 1, mode="constant", constant_values=np.nan)
+    acc = np.zeros_like(x)
+    for a in range(3):
+        for b in range(3):
+            for c in range(3):
+                wgt = WEIGHTS[a, b, c]
+                if wgt == 0.0:
+                    continue
+                nb = xp[a:a + S, b:b + R, c:c + C]
+                inside = ~np.isnan(nb)
+                d = np.where(inside, x - np.where(inside, nb, 0.0), 0.0)
+                acc += np.where(inside, wgt * d_potential(d, sigma, p), 0.0)
+    return acc
+
+
 def tv_hessian_zero(sigma: float) -> float:
     """``src/gpu/tv_hessian.cu:36-48`` + ``potential_function.cuh:111-114``:
     sum_d w_d q / (sigma^q c), evaluated in float32 like the reference."""

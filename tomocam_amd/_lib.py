@@ -39,6 +39,10 @@ SIGNATURES = {
     "tc_add_total_var": [vp, vp, C.c_int, C.c_int, C.c_int, C.c_float,
                          C.c_float, vp],
     "tc_add_tv_hessian": [c_float_p, C.c_long, C.c_float, c_float_p],
+    "tc_qggmrf_grad_free": [vp, vp, C.c_int, C.c_int, C.c_int, C.c_float,
+                            C.c_float, vp],
+    "tc_qggmrf_value_free": [vp, C.c_int, C.c_int, C.c_int, C.c_float,
+                             C.c_float, c_double_p, vp],
     "tc_run2": [vp, vp, vp, vp, C.c_int, C.c_int, C.c_double, C.c_float,
                 C.c_float, C.c_double, C.c_int, C.c_int, c_double_p,
                 c_double_p, c_int_p, vp],

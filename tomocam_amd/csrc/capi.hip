@@ -254,6 +254,24 @@ int tc_add_tv_hessian(float* h_g, long n, float sigma, float* h_value) {
   });
 }
 
+int tc_qggmrf_grad_free(const float* d_x, float* d_g, int nslices, int nrows,
+                        int ncols, float sigma, float p, void* stream) {
+  return guarded([&] {
+    TC_REQUIRE(d_x && d_g, "null argument");
+    tc::add_qggmrf_grad_free(d_x, d_g, nslices, nrows, ncols, sigma, p,
+                             as_stream(stream));
+  });
+}
+
+int tc_qggmrf_value_free(const float* d_x, int nslices, int nrows, int ncols,
+                         float sigma, float p, double* h_value, void* stream) {
+  return guarded([&] {
+    TC_REQUIRE(d_x && h_value, "null argument");
+    *h_value = tc::qggmrf_value_free(d_x, nslices, nrows, ncols, sigma, p,
+                                     as_stream(stream));
+  });
+}
+
 int tc_run2(tc_psf p, const float* d_x0, const float* d_sinoT, float* d_out,
             int nslices, int num_iters, double step, float sigma, float p_exp,
             double sino_sq, int refresh_every, int verbose, double* h_err,

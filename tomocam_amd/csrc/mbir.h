@@ -17,6 +17,18 @@ void add_total_var(const float* x, float* g, int S, int R, int C, float sigma,
 // sum_d w_d q / (sigma^q c) (src/gpu/potential_function.cuh:111-114).
 float tv_hessian_zero(float sigma);
 
+// Free-boundary qGGMRF prior for the differentiable form (no counterpart in
+// the reference, which only has the clamped stencil above):
+//   R(x) = sum over unordered in-bounds neighbour pairs {v, v+d} of
+//          w_d rho(x_v - x_{v+d}),  rho(d) = |d/s|^q / (c + |d/s|^(q-p)),
+// and its exact gradient dR/dx_v = sum_{d: v+d in bounds} w_d rho'(x_v -
+// x_{v+d}).  In the interior this is the reference stencil; at the faces the
+// pairs leaving the volume are dropped instead of clamped.
+void add_qggmrf_grad_free(const float* x, float* g, int S, int R, int C,
+                          float sigma, float p, hipStream_t stream);
+double qggmrf_value_free(const float* x, int S, int R, int C, float sigma,
+                         float p, hipStream_t stream);
+
 struct Run2Options {
   int num_iters = 100;
   double step = 1.0;

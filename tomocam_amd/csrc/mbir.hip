@@ -355,6 +355,98 @@ void add_total_var(const float* x, float* g, int S, int R, int C, float sigma,
   ktimer_end(KC_QGGMRF, stream, 3.0 * 4.0 * S * (double)R * C);
 }
 
+// ---------------------------------------------------------------------------
+// free-boundary prior and its exact gradient (differentiable form)
+// ---------------------------------------------------------------------------
+
+__global__ void reduce_strided_kernel(const double* __restrict__ p, long count,
+                                      int stride, double* out);
+
+// rho(d) = u^2 / (c + u^(q-p)), u = |d| / sigma (q = 2)
+__device__ __forceinline__ float rho_value(float d, const RhoParams& P) {
+  const float u = fabsf(d) * P.inv_sigma;
+  const float t = __builtin_amdgcn_exp2f(P.qmp * __builtin_amdgcn_logf(u));
+  return u * u * __builtin_amdgcn_rcpf(MRF_C + t);
+}
+
+// MODE 0: g[v] += sum over in-bounds neighbours of w_d rho'(x_v - x_{v+d});
+// MODE 1: per-block FP64 partial sums of the 13 forward pairs owned by v.
+// One thread per voxel; neighbours come through the caches.
+template <int MODE>
+__global__ void __launch_bounds__(256)
+qggmrf_free_kernel(const float* __restrict__ x, float* g, double* partials,
+                   int S, int R, int C, RhoParams rp) {
+  __shared__ double red[16];
+  const long nvox = (long)S * R * C;
+  const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (v < nvox) {
+    const int z = (int)(v / ((long)R * C));
+    const int rem = (int)(v - (long)z * R * C);
+    const int r = rem / C, c = rem - (rem / C) * C;
+    const float xv = x[v];
+    float sum = 0.f;
+#pragma unroll
+    for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+      for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int k = (dz != 0) + (dy != 0) + (dx != 0);
+          if (k == 0) continue;
+          // value: each unordered pair once, from its lexicographically
+          // smaller end (forward offsets)
+          const bool fwd = dz > 0 || (dz == 0 && (dy > 0 || (dy == 0 && dx > 0)));
+          if (MODE == 1 && !fwd) continue;
+          const int zz = z + dz, rr = r + dy, cc = c + dx;
+          if (zz < 0 || zz >= S || rr < 0 || rr >= R || cc < 0 || cc >= C)
+            continue;
+          const float w = k == 1 ? W_FACE : (k == 2 ? W_EDGE : W_CORNER);
+          const float d = xv - x[((long)zz * R + rr) * C + cc];
+          if (MODE == 0) {
+            sum += wdrho(d, w * rp.inv_sigma2, rp);
+          } else {
+            acc += (double)(w * rho_value(d, rp));
+          }
+        }
+    if (MODE == 0) g[v] += sum;
+  }
+  if (MODE == 1) {
+    const double tot = block_sum(acc, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+  }
+}
+
+void add_qggmrf_grad_free(const float* x, float* g, int S, int R, int C,
+                          float sigma, float p, hipStream_t stream) {
+  TC_REQUIRE(S >= 1 && R >= 1 && C >= 1, "qGGMRF: empty volume");
+  const long nvox = (long)S * R * C;
+  hipLaunchKernelGGL(qggmrf_free_kernel<0>, dim3(cdiv(nvox, 256)), dim3(256),
+                     0, stream, x, g, nullptr, S, R, C, make_rho(sigma, p));
+  TC_HIP(hipGetLastError());
+}
+
+double qggmrf_value_free(const float* x, int S, int R, int C, float sigma,
+                         float p, hipStream_t stream) {
+  TC_REQUIRE(S >= 1 && R >= 1 && C >= 1, "qGGMRF: empty volume");
+  const long nvox = (long)S * R * C;
+  const unsigned nblk = (unsigned)cdiv(nvox, 256);
+  DevBuf parts(sizeof(double) * (nblk + 1));
+  hipLaunchKernelGGL(qggmrf_free_kernel<1>, dim3(nblk), dim3(256), 0, stream,
+                     x, nullptr, parts.as<double>(), S, R, C,
+                     make_rho(sigma, p));
+  TC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(reduce_strided_kernel, dim3(1), dim3(1024), 0, stream,
+                     parts.as<double>(), (long)nblk, 1,
+                     parts.as<double>() + nblk);
+  TC_HIP(hipGetLastError());
+  double out = 0.0;
+  TC_HIP(hipMemcpyAsync(&out, parts.as<double>() + nblk, sizeof(double),
+                        hipMemcpyDeviceToHost, stream));
+  TC_HIP(hipStreamSynchronize(stream));
+  return out;
+}
+
 float tv_hessian_zero(float sigma) {
   // accumulated in float like src/gpu/tv_hessian.cu:41-46
   const float d2 = 2.0f / (sigma * sigma * MRF_C);

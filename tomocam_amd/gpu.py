@@ -143,6 +143,29 @@ def add_total_var(x: torch.Tensor, g: torch.Tensor, sigma: float,
     return g
 
 
+def qggmrf_grad_free(x: torch.Tensor, g: torch.Tensor, sigma: float,
+                     p: float = 1.2) -> torch.Tensor:
+    """g += exact gradient of the free-boundary qGGMRF prior, in place."""
+    x3 = _as3d(x)
+    g3 = _as3d(g)
+    if x3.shape != g3.shape:
+        raise ValueError("x and g must have the same shape")
+    S, R, Cc = x3.shape
+    _lib.call("tc_qggmrf_grad_free", _dev(x3, "x"), _dev(g3, "g"), S, R, Cc,
+              float(sigma), float(p), _stream())
+    return g
+
+
+def qggmrf_value_free(x: torch.Tensor, sigma: float, p: float = 1.2) -> float:
+    """Free-boundary qGGMRF prior value (FP64 accumulation)."""
+    x3 = _as3d(x)
+    S, R, Cc = x3.shape
+    v = C.c_double()
+    _lib.call("tc_qggmrf_value_free", _dev(x3, "x"), S, R, Cc, float(sigma),
+              float(p), C.byref(v), _stream())
+    return v.value
+
+
 def tv_hessian_zero(sigma: float) -> float:
     """Value gpu::add_tv_hessian would add (it never reaches the host)."""
     v = C.c_float()
