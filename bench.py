@@ -36,6 +36,13 @@ WORKLOADS = {
         slices=4, npix=256, nproj=180, iterations=50, sigma=0.1, p=1.2,
         noise=0.01, config_index=0, kind="shepp",
         label="4 slices 256x256 Shepp-Logan, 180 angles, 50 iterations"),
+    "uneven-200": dict(
+        kind="uneven", stacks=200, sizes=(256, 512, 768, 1024, 1536),
+        pmin=180, pmax=1500, smin=1, smax=16, iterations=50, sigma=0.1,
+        p=1.2, noise=0.01, config_index=4,
+        label="uneven batching: 200 seeded stacks, image size in "
+              "{256,512,768,1024,1536}, angles U{180..1500}, rays = size, "
+              "1-16 slices each, 50 iterations, qGGMRF sigma=0.1 p=1.2"),
     "mixed-size-64x2048": dict(
         slices=64, npix=2048, nproj=1501, iterations=100, sigma=0.1, p=1.2,
         noise=0.005, config_index=3,
@@ -74,6 +81,52 @@ def make_sinogram(torch, gpu, phantom, w):
     sd = w["noise"] * float(sino.max())
     sino = sino + sd * torch.randn(sino.shape, generator=g, device="cuda")
     return sino.contiguous(), ang
+
+
+def _ellipse_stack_torch(torch, n, seeds):
+    """random_ellipses (same parameter draws) rasterised on the GPU."""
+    t = (torch.arange(n, dtype=torch.float64, device="cuda") - n / 2.0) / (
+        n / 2.0)
+    yy, xx = torch.meshgrid(t, t, indexing="ij")
+    out = torch.zeros((len(seeds), n, n), dtype=torch.float32, device="cuda")
+    for i, seed in enumerate(seeds):
+        rng = np.random.default_rng(seed)
+        img = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+        for _ in range(30):
+            x0, y0 = rng.uniform(-0.5, 0.5, size=2)
+            a, b = rng.uniform(0.02, 0.4, size=2)
+            phi = rng.uniform(0.0, np.pi)
+            rho = rng.uniform(0.0, 1.0)
+            c, sn = np.cos(phi), np.sin(phi)
+            dx, dy = xx - x0, yy - y0
+            u = dx * c + dy * sn
+            v = -dx * sn + dy * c
+            img += rho * (((u / a) ** 2 + (v / b) ** 2) <= 1.0)
+        out[i] = img.to(torch.float32)
+    return out
+
+
+def make_uneven(torch, gpu, phantom, w):
+    """The uneven-batching workload: stack sizes drawn from a seeded rng;
+    slice phantoms seeded 1000*config + running slice index."""
+    rng = np.random.default_rng(1000 * w["config_index"])
+    stacks, sl = [], 0
+    for k in range(w["stacks"]):
+        n = int(rng.choice(w["sizes"]))
+        P = int(rng.integers(w["pmin"], w["pmax"] + 1))
+        S = int(rng.integers(w["smin"], w["smax"] + 1))
+        seeds = [1000 * w["config_index"] + sl + i for i in range(S)]
+        sl += S
+        img = _ellipse_stack_torch(torch, n, seeds)
+        ang = phantom.angles(P)
+        sino = gpu.project(img, ang)
+        g = torch.Generator(device="cuda")
+        g.manual_seed(7919 * k + 1)
+        sd = w["noise"] * float(sino.max())
+        sino = sino + sd * torch.randn(sino.shape, generator=g, device="cuda")
+        stacks.append((sino.contiguous(), ang, n / 2.0))
+    torch.cuda.synchronize()
+    return stacks
 
 
 def cpu_baseline(sino_host, ang, w, sample_slices, sample_iters):
@@ -136,12 +189,22 @@ def main():
         torch.cuda.synchronize()
         log("copy calibration: 1 GiB read + 1 GiB written")
         del a, c
-    sino, ang = make_sinogram(torch, gpu, phantom, w)
-    center = w["npix"] / 2.0
+    uneven = w.get("kind") == "uneven"
+    if uneven:
+        stacks = make_uneven(torch, gpu, phantom, w)
+        w["slices"] = sum(int(st[0].shape[0]) for st in stacks)
+        log(f"{len(stacks)} stacks, {w['slices']} slices")
 
-    def step():
-        return gpu.mbir(sino, ang, center, w["iterations"], w["sigma"],
-                        verbose=False)
+        def step():
+            recs, infos = gpu.mbir_batch(stacks, w["iterations"], w["sigma"])
+            return recs, infos[-1]
+    else:
+        sino, ang = make_sinogram(torch, gpu, phantom, w)
+        center = w["npix"] / 2.0
+
+        def step():
+            return gpu.mbir(sino, ang, center, w["iterations"], w["sigma"],
+                            verbose=False)
 
     for i in range(args.warmup):
         _, info = step()
@@ -188,7 +251,8 @@ def main():
         }
 
     cpu = None
-    if not args.no_cpu_baseline and int(os.environ.get("RANK", "0")) == 0:
+    if (not args.no_cpu_baseline and not uneven
+            and int(os.environ.get("RANK", "0")) == 0):
         log("cpu baseline sample ...")
         cpu = cpu_baseline(sino.cpu().numpy(), ang, w, args.cpu_sample_slices,
                            args.cpu_sample_iters)
@@ -210,13 +274,17 @@ def main():
         "dtype": "f32",
         "data": "synthetic: seeded random-ellipse phantoms projected by the "
                 "library's NUFFT projector + Gaussian noise",
-        "config": {
+        "config": ({
+            "workload": w["label"], "stacks": w["stacks"],
+            "slices": w["slices"], "iterations": w["iterations"],
+            "sigma": w["sigma"],
+        } if uneven else {
             "workload": w["label"],
             "slices": w["slices"], "npix": w["npix"], "nproj": w["nproj"],
             "iterations": w["iterations"], "sigma": w["sigma"],
             "padded_width": int(info["padded"]),
             "accepted_iterations": int(info["iterations"]),
-        },
+        }),
         "roofline": roofline,
         "kernels": kernels,
         "cpu_baseline": cpu,

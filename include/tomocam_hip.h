@@ -203,6 +203,22 @@ int tc_mbir2(const float* d_sino, const float* d_x0, float* d_recon,
              float center, int num_iters, float sigma, float tol, float xtol,
              int refresh_every, int verbose, double* h_info, void* stream);
 
+/* Uneven batching (no counterpart in the reference, which runs one stack
+ * per call): nstacks independent MBIR reconstructions of different image
+ * sizes, angle sets and slice counts in one call, each exactly tc_mbir2
+ * with an FBP initial guess.  Stack k: d_sinos[k] [nslices[k]][nproj[k]]
+ * [nrays[k]] -> d_recons[k] [nslices[k]][nrays[k]][nrays[k]], angles
+ * h_angles[k] (nproj[k] floats, host), centre centers[k].  Stacks run in
+ * order on `stream`; NUFFT plans are cached by geometry.  h_info (optional):
+ * 6 doubles per stack as for tc_mbir2.  On error the stack index is in the
+ * message and the later stacks are not run. */
+int tc_mbir2_batch(int nstacks, const float* const* d_sinos,
+                   float* const* d_recons, const int* nslices,
+                   const int* nproj, const int* nrays,
+                   const float* const* h_angles, const float* centers,
+                   int num_iters, float sigma, int refresh_every,
+                   double* h_info, void* stream);
+
 /* Same with host arrays (copies in and out, like the reference's DArray
  * interface). */
 int tc_mbir2_host(const float* h_sino, float* h_recon, int nslices, int nproj,

@@ -335,6 +335,38 @@ int tc_mbir2(const float* d_sino, const float* d_x0, float* d_recon,
   });
 }
 
+int tc_mbir2_batch(int nstacks, const float* const* d_sinos,
+                   float* const* d_recons, const int* nslices,
+                   const int* nproj, const int* nrays,
+                   const float* const* h_angles, const float* centers,
+                   int num_iters, float sigma, int refresh_every,
+                   double* h_info, void* stream) {
+  return guarded([&] {
+    TC_REQUIRE(nstacks >= 0, "mbir batch: nstacks must be >= 0");
+    TC_REQUIRE(nstacks == 0 || (d_sinos && d_recons && nslices && nproj &&
+                                nrays && h_angles && centers),
+               "null argument");
+    TC_REQUIRE(num_iters >= 0, "mbir: num_iters must be >= 0");
+    tc::Mbir2Options o;
+    o.num_iters = num_iters;
+    o.sigma = sigma;
+    o.refresh_every = refresh_every;
+    for (int k = 0; k < nstacks; ++k) {
+      try {
+        TC_REQUIRE(d_sinos[k] && d_recons[k] && h_angles[k], "null argument");
+        tc::Mbir2Result r = tc::mbir2(d_sinos[k], nullptr, d_recons[k],
+                                      nslices[k], nproj[k], nrays[k],
+                                      h_angles[k], centers[k], o,
+                                      as_stream(stream));
+        fill_info(r, h_info ? h_info + 6 * (long)k : nullptr);
+      } catch (const tc::Error& e) {
+        throw tc::Error(e.code, "stack " + std::to_string(k) + ": " +
+                                      e.what());
+      }
+    }
+  });
+}
+
 int tc_mbir2_host(const float* h_sino, float* h_recon, int nslices, int nproj,
                   int nrays, const float* h_angles, float center,
                   int num_iters, float sigma, float tol, float xtol,

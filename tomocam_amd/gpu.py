@@ -355,3 +355,47 @@ def mbir(sino: torch.Tensor, angles, center: float, num_iters: int = 100,
     keys = ("lipschitz", "step", "sino_norm", "max_sino", "padded",
             "iterations")
     return out, dict(zip(keys, info.tolist()))
+
+
+def mbir_batch(stacks, num_iters: int = 100, sigma: float = 100.0,
+               refresh_every: int = 10):
+    """Uneven batching: reconstruct many stacks in one library call.
+
+    ``stacks`` is a sequence of (sino, angles, center) with sino a CUDA
+    tensor [S, P, N] (or [P, N]); sizes, angle sets and slice counts may all
+    differ.  Each stack is exactly :func:`mbir` with the FBP initial guess.
+    Returns (list of recon tensors [S, N, N], list of info dicts)."""
+    n = len(stacks)
+    sinos, recons, angs = [], [], []
+    ns = np.zeros(max(n, 1), dtype=np.int32)
+    nps = np.zeros_like(ns)
+    nrs = np.zeros_like(ns)
+    cens = np.zeros(max(n, 1), dtype=np.float32)
+    for k, (sino, angles, center) in enumerate(stacks):
+        s3 = _as3d(sino)
+        S, P, N = s3.shape
+        a = _angles(angles)
+        if a.size != P:
+            raise ValueError(f"stack {k}: {a.size} angles for {P} projections")
+        sinos.append(s3)
+        angs.append(a)
+        recons.append(torch.empty((S, N, N), dtype=torch.float32,
+                                  device=s3.device))
+        ns[k], nps[k], nrs[k], cens[k] = S, P, N, float(center)
+    sp = (C.c_void_p * max(n, 1))(*[_dev(t, "sino").value for t in sinos])
+    rp = (C.c_void_p * max(n, 1))(*[_dev(t, "recon").value for t in recons])
+    ap = (C.c_void_p * max(n, 1))(*[a.ctypes.data for a in angs])
+    info = np.zeros(6 * max(n, 1), dtype=np.float64)
+    _lib.call("tc_mbir2_batch", n, C.cast(sp, C.c_void_p),
+              C.cast(rp, C.c_void_p),
+              ns.ctypes.data_as(C.POINTER(C.c_int)),
+              nps.ctypes.data_as(C.POINTER(C.c_int)),
+              nrs.ctypes.data_as(C.POINTER(C.c_int)),
+              C.cast(ap, C.c_void_p),
+              cens.ctypes.data_as(C.c_void_p), int(num_iters), float(sigma),
+              int(refresh_every),
+              info.ctypes.data_as(C.POINTER(C.c_double)), _stream())
+    keys = ("lipschitz", "step", "sino_norm", "max_sino", "padded",
+            "iterations")
+    infos = [dict(zip(keys, info[6 * k:6 * k + 6].tolist())) for k in range(n)]
+    return recons, infos
