@@ -42,9 +42,11 @@ def test_toeplitz_polar_psf(torch_cuda, n, P, S):
     assert abs(xtx - np.sum(x * ref)) <= 1e-5 * abs(np.sum(x * ref))
 
 
-@pytest.mark.parametrize("n,S", [(181, 2), (362, 2), (726, 1), (1454, 3), (2908, 1)])
+@pytest.mark.parametrize("n,S", [(181, 2), (362, 2), (726, 1), (1090, 2),
+                                 (1454, 3), (2180, 1), (2908, 1)])
 def test_toeplitz_sizes(torch_cuda, n, S):
-    """FFT sizes of the workloads (M = 2n-1 rounded to a smooth size)."""
+    """FFT sizes of the workloads: M = 378 (generic kernels), 750, 1458,
+    2250, 2916, 4500, 5832 (compile-time plans)."""
     torch = torch_cuda
     from tomocam_amd import gpu
     ps = _even_psf(n, 7)

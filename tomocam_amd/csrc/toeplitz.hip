@@ -11,6 +11,21 @@ namespace tc {
 
 // lines per workgroup of the compile-time plan for FFT length M (0: none)
 int toeplitz_ct_lpw(int M);
+extern const int kToeplitzCtSizes[];
+extern const int kToeplitzCtCount;
+
+// FFT length for an n x n Toeplitz product: any even length >= 2n - 1 gives
+// the same result; take the smallest compile-time plan size >= 2n - 1 when
+// it is within 10 % of the smallest even 2,3,5,7-smooth length, else that.
+static int toeplitz_fft_length(int n) {
+  const int m = 2 * n - 1;
+  const int smooth = fft_next_smooth(m, true);
+  for (int i = 0; i < kToeplitzCtCount; ++i) {
+    const int c = kToeplitzCtSizes[i];
+    if (c >= m) return c <= smooth + smooth / 10 ? c : smooth;
+  }
+  return smooth;
+}
 
 // ---------------------------------------------------------------------------
 // PSF spectrum
@@ -45,7 +60,7 @@ __global__ void psf_extract_kernel(const float2* __restrict__ c, float* spec,
 
 Psf::Psf(const float* d_psf, int n, hipStream_t stream) : n_(n) {
   TC_REQUIRE(n >= 2, "PSF: image width must be >= 2");
-  M_ = fft_next_smooth(2 * n - 1, true);
+  M_ = toeplitz_fft_length(n);
   H_ = M_ / 2 + 1;
   rp_ = n + (n & 1);  // rows of the workspace: whole pairs
   const size_t n1 = 2 * (size_t)n - 1;

@@ -36,7 +36,9 @@ struct TPlan {
 
 using Plan750 = TPlan<750, 30, 10, Radices<10, 15, 5>, Radices<5, 15, 10>>;
 using Plan1458 = TPlan<1458, 18, 3, Radices<18, 9, 9>, Radices<9, 9, 18>>;
+using Plan2250 = TPlan<2250, 30, 3, Radices<10, 15, 15>, Radices<15, 15, 10>>;
 using Plan2916 = TPlan<2916, 18, 3, Radices<18, 18, 9>, Radices<9, 18, 18>>;
+using Plan4500 = TPlan<4500, 30, 2, Radices<10, 15, 30>, Radices<30, 15, 10>>;
 using Plan5832 = TPlan<5832, 18, 1, Radices<18, 18, 18>, Radices<18, 18, 18>>;
 
 // Wt element (slice s, column k1, row): layout [s][k1][rp], so the column
@@ -425,11 +427,17 @@ static void run_ct(const float* x, const float* b, float* out, float2* wt,
   }
 }
 
+// FFT lengths with compile-time plans, ascending
+extern const int kToeplitzCtSizes[] = {750, 1458, 2250, 2916, 4500, 5832};
+extern const int kToeplitzCtCount = 6;
+
 int toeplitz_ct_lpw(int M) {
   switch (M) {
     case 750: return Plan750::LPW;
     case 1458: return Plan1458::LPW;
+    case 2250: return Plan2250::LPW;
     case 2916: return Plan2916::LPW;
+    case 4500: return Plan4500::LPW;
     case 5832: return Plan5832::LPW;
     default: return 0;
   }
@@ -449,8 +457,16 @@ bool toeplitz_apply_ct(int M, const float* x, const float* b, float* out,
       run_ct<Plan1458>(x, b, out, wt, spec, n, H, rp, nslices, mode, partials,
                        stream, img_b, spec_b, sc);
       return true;
+    case 2250:
+      run_ct<Plan2250>(x, b, out, wt, spec, n, H, rp, nslices, mode, partials,
+                       stream, img_b, spec_b, sc);
+      return true;
     case 2916:
       run_ct<Plan2916>(x, b, out, wt, spec, n, H, rp, nslices, mode, partials,
+                       stream, img_b, spec_b, sc);
+      return true;
+    case 4500:
+      run_ct<Plan4500>(x, b, out, wt, spec, n, H, rp, nslices, mode, partials,
                        stream, img_b, spec_b, sc);
       return true;
     case 5832:
