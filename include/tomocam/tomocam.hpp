@@ -372,4 +372,22 @@ DArray<T> mbir2(std::optional<DArray<T>> guess, const DArray<T>& sino,
   return out;
 }
 
+// src/mbir.cpp:38-119 (the NUFFT normal-operator variant; x0 required)
+template <typename T>
+DArray<T> mbir(DArray<T>& x0, DArray<T>& sino, std::vector<T> angles,
+               T center, int num_iters, T sigma, T tol, T xtol) {
+  detail::require_float<T>();
+  const int S = sino.nslices(), P = sino.nrows(), N = sino.ncols();
+  detail::Dev d_sino(sino.begin(), sino.size());
+  detail::Dev d_x0(x0.begin(), x0.size());
+  DArray<T> out(dim3_t(S, N, N));
+  detail::Dev d_out(out.size());
+  detail::check(tc_mbir(d_sino.get(), d_x0.get(), d_out.get(), S, P, N,
+                        angles.data(), center, num_iters, sigma, tol, xtol, 1,
+                        nullptr, nullptr),
+                "mbir");
+  d_out.to_host(out.begin());
+  return out;
+}
+
 }  // namespace tomocam

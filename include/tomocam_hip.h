@@ -203,6 +203,18 @@ int tc_mbir2(const float* d_sino, const float* d_x0, float* d_recon,
              float center, int num_iters, float sigma, float tol, float xtol,
              int refresh_every, int verbose, double* h_info, void* stream);
 
+/* tomocam::mbir (src/mbir.cpp:38-119): the NUFFT normal-operator MBIR,
+ * without a PSF.  Normalises by (sino - min) / (max - min), pads and
+ * centres, b = P^T y, step = min(1, 1 / max(Re(A^H A 1)/n^3)), then the
+ * reference's run2 with gradient Re(A^H A y)/n^3 - b + R'(y) and error
+ * ||P v - y||.  d_x0 [nslices][nrays][nrays] (device) is required, as in
+ * the reference; a sinogram with max == min returns d_x0 unchanged.
+ * h_info as for tc_mbir2 (padded width 0 for the unchanged return). */
+int tc_mbir(const float* d_sino, const float* d_x0, float* d_recon,
+            int nslices, int nproj, int nrays, const float* h_angles,
+            float center, int num_iters, float sigma, float tol, float xtol,
+            int verbose, double* h_info, void* stream);
+
 /* Uneven batching (no counterpart in the reference, which runs one stack
  * per call): nstacks independent MBIR reconstructions of different image
  * sizes, angle sets and slice counts in one call, each exactly tc_mbir2

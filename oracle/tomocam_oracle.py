@@ -795,6 +795,44 @@ def mbir2(sino: np.ndarray, angles: np.ndarray, center: float, num_iters: int,
     return postproc(rec, st.nrays)
 
 
+def mbir_nufft(x0: np.ndarray, sino: np.ndarray, angles: np.ndarray,
+               center: float, num_iters: int, sigma: float, p: float = 1.2,
+               exact: bool = False, log: Optional[Run2Log] = None):
+    """``src/mbir.cpp:38-119``: the NUFFT normal-operator MBIR (no PSF).
+
+    Normalise by (y - min) / (max - min) (x0 returned unchanged if
+    max == min), pad the guess, b = P^T y, step = min(1, 1/max(Re(A^H A 1)
+    / n^3)), then :func:`run2` with gradient Re(A^H A y)/n^3 - b + R'(y) and
+    error ||P v - y||."""
+    sino = np.asarray(sino, dtype=np.float64)
+    if sino.ndim == 2:
+        sino = sino[None]
+    x0 = np.asarray(x0, dtype=np.float64)
+    if x0.ndim == 2:
+        x0 = x0[None]
+    maxv, minv = sino.max(), sino.min()
+    if maxv == minv:
+        return x0
+    sino2 = (sino - minv) / (maxv - minv)
+    nrays = sino.shape[-1]
+    sino2 = preproc(sino2, center)
+    x0p = pad2d(x0, sino2.shape[-1] - x0.shape[-1])
+    n2 = sino2.shape[-1]
+    sinoT = backproject(sino2, angles, exact=exact)
+    ones = np.ones((1, n2, n2))
+    L = float(np.max(gradient_nufft(ones, np.zeros_like(ones), angles, exact)))
+    step = min(1.0, 1.0 / L)
+
+    def grad_fn(x):
+        return gradient_nufft(x, sinoT, angles, exact) + qggmrf_grad(x, sigma, p)
+
+    def err_fn(x):
+        return math.sqrt(function_value(x, sino2, angles, exact))
+
+    sol = run2(x0p, grad_fn, err_fn, num_iters, step, log)
+    return postproc(sol, nrays)
+
+
 def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
     a = np.asarray(a)
     b = np.asarray(b)

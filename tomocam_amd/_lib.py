@@ -58,6 +58,9 @@ SIGNATURES = {
     "tc_mbir2": [vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, C.c_float,
                  C.c_int, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int,
                  c_double_p, vp],
+    "tc_mbir": [vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, C.c_float,
+                C.c_int, C.c_float, C.c_float, C.c_float, C.c_int, c_double_p,
+                vp],
     "tc_mbir2_batch": [C.c_int, vp, vp, c_int_p, c_int_p, c_int_p, vp, vp,
                        C.c_int, C.c_float, C.c_int, c_double_p, vp],
     "tc_mbir2_host": [vp, vp, C.c_int, C.c_int, C.c_int, vp, C.c_float,

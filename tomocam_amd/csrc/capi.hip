@@ -335,6 +335,26 @@ int tc_mbir2(const float* d_sino, const float* d_x0, float* d_recon,
   });
 }
 
+int tc_mbir(const float* d_sino, const float* d_x0, float* d_recon,
+            int nslices, int nproj, int nrays, const float* h_angles,
+            float center, int num_iters, float sigma, float tol, float xtol,
+            int verbose, double* h_info, void* stream) {
+  return guarded([&] {
+    (void)tol;
+    (void)xtol;
+    TC_REQUIRE(d_sino && d_x0 && d_recon && h_angles, "null argument");
+    TC_REQUIRE(num_iters >= 0, "mbir: num_iters must be >= 0");
+    tc::Mbir2Options o;
+    o.num_iters = num_iters;
+    o.sigma = sigma;
+    o.verbose = verbose;
+    tc::Mbir2Result r = tc::mbir_nufft(d_sino, d_x0, d_recon, nslices, nproj,
+                                       nrays, h_angles, center, o,
+                                       as_stream(stream));
+    fill_info(r, h_info);
+  });
+}
+
 int tc_mbir2_batch(int nstacks, const float* const* d_sinos,
                    float* const* d_recons, const int* nslices,
                    const int* nproj, const int* nrays,

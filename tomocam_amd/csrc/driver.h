@@ -47,4 +47,14 @@ Mbir2Result mbir2(const float* sino, const float* x0, float* recon, int S,
                   int P, int N, const float* h_angles, float center,
                   const Mbir2Options& o, hipStream_t stream);
 
+// tomocam::mbir (src/mbir.cpp:38-119): the NUFFT normal-operator variant.
+// Normalises by (sino - min) / (max - min), pads the given guess x0 [S][N][N]
+// (required), takes the step from max Re(A^H A 1)/n^3, and runs the
+// reference's run2 with gradient(y) = Re(A^H A y)/n^3 - P^T y_d + R'(y) and
+// error(v) = ||P v - y_d|| (projector, not the Toeplitz expansion).  A
+// sinogram with max == min returns x0 unchanged (res.padded = 0).
+Mbir2Result mbir_nufft(const float* sino, const float* x0, float* recon, int S,
+                       int P, int N, const float* h_angles, float center,
+                       const Mbir2Options& o, hipStream_t stream);
+
 }  // namespace tc

@@ -1,8 +1,10 @@
 // Pre/post-processing kernels and the device-resident mbir2 pipeline.
 #include <cmath>
 #include <cstdio>
+#include <utility>
 
 #include "driver.h"
+#include "mbir.h"
 #include "nufft.h"
 #include "toeplitz.h"
 #include "vecops.h"
@@ -183,6 +185,150 @@ Mbir2Result mbir2(const float* sino, const float* x0, float* recon, int S,
 
   // crop (src/mbir2.cpp:144)
   postproc(bsol.as<float>(), recon, S, n2, N, stream);
+  TC_HIP(hipStreamSynchronize(stream));
+  return res;
+}
+
+// out = (1 + beta) a - beta b, i.e. a + beta (a - b)
+__global__ void extrapolate_kernel(const float* a, const float* b, float* out,
+                                   long n, float beta) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = fmaf(beta, a[i] - b[i], a[i]);
+}
+
+// out = a - s g
+__global__ void step_kernel(const float* a, const float* g, float* out,
+                            long n, float s) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = fmaf(-s, g[i], a[i]);
+}
+
+Mbir2Result mbir_nufft(const float* sino, const float* x0, float* recon, int S,
+                       int P, int N, const float* h_angles, float center,
+                       const Mbir2Options& o, hipStream_t stream) {
+  TC_REQUIRE(S >= 1 && P >= 1 && N >= 2, "mbir: bad sinogram shape");
+  TC_REQUIRE(h_angles != nullptr && x0 != nullptr,
+             "mbir: angles and an initial guess are required");
+  Mbir2Result res;
+  const long nsino = (long)S * P * N;
+  const size_t img_bytes = sizeof(float) * (size_t)S * N * N;
+
+  // normalise to [0, 1] (src/mbir.cpp:43-54)
+  float mx = 0.f, mn = 0.f;
+  minmax(sino, nsino, &mx, &mn, stream);
+  res.max_sino = mx;
+  if (mx == mn) {
+    TC_HIP(hipMemcpyAsync(recon, x0, img_bytes, hipMemcpyDeviceToDevice,
+                          stream));
+    TC_HIP(hipStreamSynchronize(stream));
+    return res;
+  }
+  DevBuf bsino2(sizeof(float) * nsino);
+  affine(sino, bsino2.as<float>(), nsino, -mn, 1.0f / (mx - mn), stream);
+
+  // pad and centre (src/mbir.cpp:56-60)
+  int npad = 0, off = 0;
+  pad_amount(N, center, &npad, &off);
+  const int n2 = N + 2 * npad;
+  res.npad = npad;
+  res.padded = n2;
+  const long ns2 = (long)S * P * n2;
+  DevBuf bsinop(sizeof(float) * ns2);
+  preproc(bsino2.as<float>(), bsinop.as<float>(), S, P, N, npad, off, stream);
+  bsino2.release();
+  const long nimg = (long)S * n2 * n2;
+  const size_t vb = sizeof(float) * (size_t)nimg;
+  DevBuf bsol(vb), bx(vb), by(vb), bg(vb), bc(vb), bsinoT(vb);
+  pad2d(x0, bsol.as<float>(), S, N, n2 - N, stream);
+  TC_HIP(hipMemcpyAsync(bx.as<void>(), bsol.as<void>(), vb,
+                        hipMemcpyDeviceToDevice, stream));
+
+  // P^T y (src/mbir.cpp:68)
+  backproject(bsinop.as<float>(), bsinoT.as<float>(), S, P, n2, h_angles,
+              false, stream);
+  res.sino_norm = dot(bsinop.as<float>(), bsinop.as<float>(), ns2, stream);
+
+  PolarNufft& nu = nufft_plan(h_angles, P, n2, n2);
+  const float inv_n3 = (float)(1.0 / ((double)n2 * n2 * n2));
+  DevBuf bcoef(sizeof(float2) * (size_t)ns2);
+  DevBuf bproj(sizeof(float) * ns2);
+  // Re(A^H A v)/n^3 - b (+ R'(v))  (src/gradient.cpp:35-103)
+  auto gradient = [&](const float* v, float* g, int nsl, const float* b) {
+    nu.type2_real(v, bcoef.as<float2>(), nsl, stream);
+    nu.type1_real(bcoef.as<float2>(), g, nsl, inv_n3, stream);
+    if (b) affine_sub(g, b, g, (long)nsl * n2 * n2, stream);
+  };
+  // ||P v - y||  (src/error.cpp:36-88)
+  auto error = [&](const float* v) {
+    project(v, bproj.as<float>(), S, n2, h_angles, P, stream);
+    return std::sqrt(diff_norm2(bproj.as<float>(), bsinop.as<float>(), ns2,
+                                stream));
+  };
+
+  // L = max(gradient(ones, 0)); the Hessian term stays on a device copy in
+  // the reference (src/mbir.cpp:83-96)
+  {
+    const long nn = (long)n2 * n2;
+    hipLaunchKernelGGL(fill_kernel, dim3(cdiv(nn, 256)), dim3(256), 0, stream,
+                       by.as<float>(), nn, 1.0f);
+    TC_HIP(hipGetLastError());
+    gradient(by.as<float>(), bg.as<float>(), 1, nullptr);
+    float lmax = 0.f, lmin = 0.f;
+    minmax(bg.as<float>(), nn, &lmax, &lmin, stream);
+    res.lipschitz = lmax;
+  }
+  double step = 1.0 / res.lipschitz;
+  if (step > 1.0) step = 1.0;
+  res.step = step;
+
+  // Optimizer::run2 (src/optimize.h:96-154), restated directly: four
+  // projector evaluations per attempt as in the reference
+  float* sol = bsol.as<float>();
+  float* x = bx.as<float>();
+  float* y = by.as<float>();
+  float* g = bg.as<float>();
+  float* cand = bc.as<float>();
+  const unsigned nb = (unsigned)cdiv(nimg, 256);
+  double t = 1.0, tnew = 1.0;
+  const double step0 = step;
+  for (int iter = 0; iter < o.num_iters; ++iter) {
+    double xerr = 0.0;
+    for (;;) {
+      const double beta = tnew * (1.0 / t - 1.0);
+      tnew = 0.5 * (std::sqrt(t * t * t * t + 4.0 * t * t) - t * t);
+      hipLaunchKernelGGL(extrapolate_kernel, dim3(nb), dim3(256), 0, stream,
+                         sol, x, y, nimg, (float)beta);
+      TC_HIP(hipGetLastError());
+      gradient(y, g, S, bsinoT.as<float>());
+      add_total_var(y, g, S, n2, n2, o.sigma, o.p, stream);
+      hipLaunchKernelGGL(step_kernel, dim3(nb), dim3(256), 0, stream, y, g,
+                         cand, nimg, (float)step);
+      TC_HIP(hipGetLastError());
+      const double fx = error(cand);
+      const double fy = error(y);
+      const double gy = 0.5 * step * dot(g, g, nimg, stream);
+      std::swap(sol, cand);  // sol takes the candidate on every attempt
+      ++res.stats.attempts;
+      if (fx > fy + gy) {
+        step *= 0.9;
+        ++res.stats.backtracks;
+        continue;
+      }
+      xerr = diff_norm2(sol, x, nimg, stream);
+      TC_HIP(hipMemcpyAsync(x, sol, vb, hipMemcpyDeviceToDevice, stream));
+      step = step0;
+      t = tnew;
+      ++res.stats.iterations;
+      break;
+    }
+    const double e = error(sol);
+    res.stats.last_error = e;
+    res.stats.last_xerr = std::sqrt(xerr);
+    if (o.verbose)
+      std::fprintf(stdout, "iter: %4d, error: %5.4e, x-err: %5.4e\n", iter, e,
+                   std::sqrt(xerr));
+  }
+  postproc(sol, recon, S, n2, N, stream);
   TC_HIP(hipStreamSynchronize(stream));
   return res;
 }

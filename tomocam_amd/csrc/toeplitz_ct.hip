@@ -188,19 +188,27 @@ ct_cols(float2* __restrict__ wt, const float* __restrict__ spec,
       v[i * R1 + r] = (active && idx < n) ? wt[wt_at(s, kc, idx, H, rp)]
                                           : make_float2(0.f, 0.f);
     }
+  // the spectrum values this thread multiplies by, loaded up front so
+  // their latency overlaps the forward transform
+  float hs[E];
+  {
+    const float* sp = spec + (long)kc * L;
+#pragma unroll
+    for (int i = 0; i < E / RS; ++i)
+#pragma unroll
+      for (int r = 0; r < RS; ++r) hs[i * RS + r] = sp[t + i * T + r * (L / RS)];
+  }
   float2* ln = sm + line * L;
   F::run(v, ln, t, active, tw);
   double acc = 0.0;
   if (active) {
-    const float* sp = spec + (long)k1 * L;
     const float wgt = (k1 == 0 || 2 * k1 == L) ? 1.f : 2.f;
     float facc = 0.f;
 #pragma unroll
     for (int i = 0; i < E / RS; ++i)
 #pragma unroll
       for (int r = 0; r < RS; ++r) {
-        const int k2 = t + i * T + r * (L / RS);
-        const float h = sp[k2];
+        const float h = hs[i * RS + r];
         const float2 q = v[i * RS + r];
         facc += h * (q.x * q.x + q.y * q.y);
         v[i * RS + r] = cscale(q, h);
@@ -286,7 +294,8 @@ ct_rows_inv(const float2* __restrict__ wt, const float* bvec, float* out,
   const long oa = (long)s * n * n + (long)ra * n;
   const bool hasa = ra < n, hasb = ra + 1 < n;
   // epilogue operands (b, or T sol and T x) all loaded before any store,
-  // which the compiler could not otherwise move past (out may alias Tsol)
+  // which the compiler could not otherwise move past (out may alias Tsol).
+  // (Issuing them before the transform measured slower: more registers.)
   float2 e0[MODE != 0 ? E : 1], e1[MODE == 3 ? E : 1];
   if constexpr (MODE != 0) {
     const float* src0 = MODE == 1 ? bvec : cp.Tsol;

@@ -357,6 +357,33 @@ def mbir(sino: torch.Tensor, angles, center: float, num_iters: int = 100,
     return out, dict(zip(keys, info.tolist()))
 
 
+def mbir_nufft(sino: torch.Tensor, angles, center: float,
+               x0: torch.Tensor, num_iters: int = 100, sigma: float = 100.0,
+               tol: float = 1e-4, xtol: float = 1e-4,
+               verbose: bool = False):
+    """tomocam::mbir (src/mbir.cpp:38-119), the NUFFT normal-operator MBIR
+    without a PSF: sinogram [S, P, N] and initial guess x0 [S, N, N] (both
+    device) -> (recon [S, N, N], info)."""
+    sino = _as3d(sino)
+    S, P, N = sino.shape
+    ang = _angles(angles)
+    if ang.size != P:
+        raise ValueError(f"got {ang.size} angles for {P} projections")
+    x0 = _as3d(x0)
+    if tuple(x0.shape) != (S, N, N):
+        raise ValueError(f"x0 must be [{S}, {N}, {N}]")
+    out = torch.empty((S, N, N), dtype=torch.float32, device=sino.device)
+    info = np.zeros(6, dtype=np.float64)
+    _lib.call("tc_mbir", _dev(sino, "sino"), _dev(x0, "x0"), _dev(out, "recon"),
+              S, P, N, ang.ctypes.data_as(C.c_void_p), float(center),
+              int(num_iters), float(sigma), float(tol), float(xtol),
+              int(bool(verbose)), info.ctypes.data_as(_lib.c_double_p),
+              _stream())
+    keys = ("lipschitz", "step", "sino_norm", "max_sino", "padded",
+            "iterations")
+    return out, dict(zip(keys, info.tolist()))
+
+
 def mbir_batch(stacks, num_iters: int = 100, sigma: float = 100.0,
                refresh_every: int = 10):
     """Uneven batching: reconstruct many stacks in one library call.
