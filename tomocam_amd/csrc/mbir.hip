@@ -73,7 +73,10 @@ struct QArgs {
 // instead of 26 QROWS.
 constexpr int QLANES = 64;
 constexpr int QCOLS = QLANES - 2;
-constexpr int QROWS = 8;
+#ifndef TC_QROWS
+#define TC_QROWS 8
+#endif
+constexpr int QROWS = TC_QROWS;
 constexpr int QWAVES = 4;  // strips per workgroup, stacked in y
 constexpr int QJ = QROWS + 2;
 // line-search dot products per workgroup (MBIR mode):

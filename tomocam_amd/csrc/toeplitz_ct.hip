@@ -25,11 +25,13 @@ struct Xform<L, E, DIR, Radices<Rs...>> {
 };
 
 // Plans: forward radices Fwd, inverse radices Inv = reversed Fwd, LPW lines
-// per workgroup.
-template <int L_, int E_, int LPW_, class Fwd_, class Inv_>
+// per workgroup in the row passes and LPWC in the column pass.
+template <int L_, int E_, int LPW_, class Fwd_, class Inv_, int LPWC_ = LPW_>
 struct TPlan {
   static constexpr int L = L_, E = E_, LPW = LPW_, T = L_ / E_;
   static constexpr int NT = LPW_ * (L_ / E_);
+  static constexpr int LPWC = LPWC_;
+  static constexpr int NTC = LPWC_ * (L_ / E_);
   using Fwd = Fwd_;
   using Inv = Inv_;
 };
@@ -37,7 +39,19 @@ struct TPlan {
 using Plan750 = TPlan<750, 30, 10, Radices<10, 15, 5>, Radices<5, 15, 10>>;
 using Plan1458 = TPlan<1458, 18, 3, Radices<18, 9, 9>, Radices<9, 9, 18>>;
 using Plan2250 = TPlan<2250, 30, 3, Radices<10, 15, 15>, Radices<15, 15, 10>>;
-using Plan2916 = TPlan<2916, 18, 3, Radices<18, 18, 9>, Radices<9, 18, 18>>;
+#ifndef TC_P2916_E
+#define TC_P2916_E 18
+#endif
+#ifndef TC_P2916_LPW
+#define TC_P2916_LPW 3
+#endif
+#ifndef TC_P2916_LPWC
+#define TC_P2916_LPWC 1
+#endif
+// Column pass: one column per workgroup measured 5 % faster than three at
+// this length; at 2250 three per workgroup stay 20 % faster than one.
+using Plan2916 = TPlan<2916, TC_P2916_E, TC_P2916_LPW, Radices<18, 18, 9>,
+                       Radices<9, 18, 18>, TC_P2916_LPWC>;
 using Plan4500 = TPlan<4500, 30, 2, Radices<10, 15, 30>, Radices<30, 15, 10>>;
 using Plan5832 = TPlan<5832, 18, 1, Radices<18, 18, 18>, Radices<18, 18, 18>>;
 
@@ -157,13 +171,13 @@ ct_rows_fwd(const float* __restrict__ x, float2* __restrict__ wt,
 // ---------------------------------------------------------------------------
 
 template <class P>
-__global__ void __launch_bounds__(P::NT)
+__global__ void __launch_bounds__(P::NTC)
 ct_cols(float2* __restrict__ wt, const float* __restrict__ spec,
         const float2* __restrict__ tw, int n, int H, int rp, int nbps,
         double* partials) {
   extern __shared__ float2 sm[];
   __shared__ double red[16];
-  constexpr int L = P::L, E = P::E, T = P::T, LPW = P::LPW;
+  constexpr int L = P::L, E = P::E, T = P::T, LPW = P::LPWC;
   using F = Xform<L, E, -1, typename P::Fwd>;
   using I = Xform<L, E, 1, typename P::Inv>;
   constexpr int R1 = F::first, RS = F::last;
@@ -369,9 +383,10 @@ static void run_ct(const float* x, const float* b, float* out, float2* wt,
                    double img_b, double spec_b, const Psf::StepCommit* sc) {
   const float2* tw = fft_length(P::L).plan().tw;
   const size_t smem = sizeof(float2) * P::LPW * P::L;
+  const size_t smemc = sizeof(float2) * P::LPWC * P::L;
   const int npairs = (n + 1) / 2;
   const int nbr = (int)cdiv(npairs, P::LPW);  // row-pass blocks per slice
-  const int nbc = (int)cdiv(H, P::LPW);       // column-pass blocks per slice
+  const int nbc = (int)cdiv(H, P::LPWC);      // column-pass blocks per slice
   const long img = (long)n * n;
   const int cm = sc ? (sc->xp ? 2 : 1) : 0;
   const int em = cm ? cm + 1 : mode;  // epilogue mode of pass C
@@ -407,9 +422,9 @@ static void run_ct(const float* x, const float* b, float* out, float2* wt,
     }
     {
       auto k = ct_cols<P>;
-      allow_lds((const void*)k, smem);
+      allow_lds((const void*)k, smemc);
       ktimer_begin(KC_TOEPLITZ_COLS, stream);
-      hipLaunchKernelGGL(k, dim3(nbc * ns), dim3(P::NT), smem, stream, wt,
+      hipLaunchKernelGGL(k, dim3(nbc * ns), dim3(P::NTC), smemc, stream, wt,
                          spec, tw, n, H, rp, nbc,
                          partials ? partials + (long)s0 * nbc : nullptr);
       TC_HIP(hipGetLastError());
@@ -440,14 +455,15 @@ static void run_ct(const float* x, const float* b, float* out, float2* wt,
 extern const int kToeplitzCtSizes[] = {750, 1458, 2250, 2916, 4500, 5832};
 extern const int kToeplitzCtCount = 6;
 
+// column-pass lines per workgroup (0: no compile-time plan for M)
 int toeplitz_ct_lpw(int M) {
   switch (M) {
-    case 750: return Plan750::LPW;
-    case 1458: return Plan1458::LPW;
-    case 2250: return Plan2250::LPW;
-    case 2916: return Plan2916::LPW;
-    case 4500: return Plan4500::LPW;
-    case 5832: return Plan5832::LPW;
+    case 750: return Plan750::LPWC;
+    case 1458: return Plan1458::LPWC;
+    case 2250: return Plan2250::LPWC;
+    case 2916: return Plan2916::LPWC;
+    case 4500: return Plan4500::LPWC;
+    case 5832: return Plan5832::LPWC;
     default: return 0;
   }
 }
