@@ -10,8 +10,10 @@ padding, P^T y, PSF, step size, 100 iterations, crop) on a sinogram already
 resident in HBM.
 
 Prints one JSON line with the throughput, the roofline of the dominant
-kernel (HIP-event timed inside the timed region) and a CPU baseline (the
-FP32 CPU restatement of the same algorithm on a bounded sample).
+kernel (HIP-event timed inside the timed region), a CPU baseline (the FP32
+CPU restatement of the same algorithm on a bounded sample) and the
+mixed-precision form against plain FP32 on 64 x 2048^2 (one timed call each,
+sigma = 100).
 """
 
 import argparse
@@ -47,6 +49,16 @@ WORKLOADS = {
         slices=64, npix=2048, nproj=1501, iterations=100, sigma=0.1, p=1.2,
         noise=0.005, config_index=3,
         label="64 slices 2048x2048, 1501 angles, 100 iterations (plain FP32)"),
+    # the mixed-precision comparison: sigma = 100 (the reference's default
+    # smoothness 0.01), where the iteration converges and relative errors
+    # between precisions are meaningful
+    "mixed-prec-64x2048": dict(
+        slices=64, npix=2048, nproj=1501, iterations=100, sigma=100.0, p=1.2,
+        noise=0.005, config_index=3, precision="mixed",
+        label="mixed precision (FP16 gradient and Toeplitz intermediates, "
+              "FP32 FFTs and state, FP64 dots, FP32 refresh every 10): 64 "
+              "slices 2048x2048, 1501 angles x 2048 rays, 100 iterations, "
+              "qGGMRF sigma=100 p=1.2"),
 }
 
 PEAK_HBM_GBS = 8000.0
@@ -72,9 +84,17 @@ def log(msg):
 
 def make_sinogram(torch, gpu, phantom, w):
     kind = w.get("kind", "ellipses")
-    img = phantom.phantom_stack(w["npix"], w["slices"], kind, w["config_index"])
     ang = phantom.angles(w["nproj"])
-    sino = gpu.project(torch.from_numpy(img).cuda(), ang)
+    if kind == "ellipses" and w["npix"] * w["npix"] * w["slices"] > (1 << 26):
+        # large stacks: the same phantoms (phantom_stack's seeds) rasterised
+        # on the GPU
+        seeds = [1000 * w["config_index"] + s for s in range(w["slices"])]
+        img = _ellipse_stack_torch(torch, w["npix"], seeds)
+    else:
+        img = torch.from_numpy(phantom.phantom_stack(
+            w["npix"], w["slices"], kind, w["config_index"])).cuda()
+    sino = gpu.project(img, ang)
+    del img
     torch.cuda.synchronize()
     g = torch.Generator(device="cuda")
     g.manual_seed(1000 * w["config_index"] + 7)
@@ -147,6 +167,39 @@ def cpu_baseline(sino_host, ang, w, sample_slices, sample_iters):
     }
 
 
+def mixed_compare(torch, gpu, phantom, warmup):
+    """Mixed precision against plain FP32 on the 64 x 2048^2 workload: one
+    timed call of each (after `warmup` short calls priming the plans), their
+    slices/s and the relative L2 difference of the reconstructions."""
+    w = dict(WORKLOADS["mixed-prec-64x2048"])
+    sino, ang = make_sinogram(torch, gpu, phantom, w)
+    center = w["npix"] / 2.0
+    res, recs = {}, {}
+    for prec in ("fp32", "mixed"):
+        for _ in range(warmup):
+            gpu.mbir(sino, ang, center, 2, w["sigma"], precision=prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rec, info = gpu.mbir(sino, ang, center, w["iterations"], w["sigma"],
+                             precision=prec)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        res[prec] = (w["slices"] / dt, info)
+        recs[prec] = rec
+        log(f"{prec}: {w['slices'] / dt:.3f} slices/s ({dt:.2f} s)")
+    d = (recs["mixed"].double() - recs["fp32"].double()).norm()
+    rel = float(d / recs["fp32"].double().norm())
+    return {
+        "workload": w["label"],
+        "fp32_slices_per_s": res["fp32"][0],
+        "mixed_slices_per_s": res["mixed"][0],
+        "speedup": res["mixed"][0] / res["fp32"][0],
+        "relL2_mixed_vs_fp32": rel,
+        "saturated_slice_attempts": int(res["mixed"][1]["saturated"]),
+        "accepted_iterations": int(res["mixed"][1]["iterations"]),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,6 +210,9 @@ def main():
     ap.add_argument("--cpu-sample-slices", type=int, default=8)
     ap.add_argument("--cpu-sample-iters", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision vs FP32 comparison on the "
+                         "64 x 2048^2 workload")
     ap.add_argument("--no-ktimer", action="store_true",
                     help="do not bracket kernels with HIP events")
     ap.add_argument("--iterations", type=int, default=None,
@@ -202,9 +258,11 @@ def main():
         sino, ang = make_sinogram(torch, gpu, phantom, w)
         center = w["npix"] / 2.0
 
+        prec = w.get("precision", "fp32")
+
         def step():
             return gpu.mbir(sino, ang, center, w["iterations"], w["sigma"],
-                            verbose=False)
+                            verbose=False, precision=prec)
 
     for i in range(args.warmup):
         _, info = step()
@@ -258,6 +316,14 @@ def main():
                            args.cpu_sample_iters)
         log(f"cpu baseline {cpu['value']:.4g} slices/s on {cpu['cores']} threads")
 
+    mixed = None
+    if not args.no_mixed and args.workload == "plain-fp32-32x1024":
+        if not uneven:
+            del sino
+        torch.cuda.empty_cache()
+        log("mixed precision vs FP32 on 64 x 2048^2 ...")
+        mixed = mixed_compare(torch, gpu, phantom, max(args.warmup, 1))
+
     value = w["slices"] * args.steps / elapsed
     out = {
         "metric": "MBIR slices/s (mixed-prec vs plain FP32) at relL2≤1e-4 "
@@ -271,7 +337,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f16 storage / f32 compute" if w.get("precision") == "mixed"
+                 else "f32",
         "data": "synthetic: seeded random-ellipse phantoms projected by the "
                 "library's NUFFT projector + Gaussian noise",
         "config": ({
@@ -288,6 +355,7 @@ def main():
         "roofline": roofline,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "mixed_precision": mixed,
     }
     print(json.dumps(out), flush=True)
 

@@ -125,6 +125,15 @@ int tc_psf_info(tc_psf p, int* n, int* fft_size);
 int tc_toeplitz_apply(tc_psf p, const float* d_x, float* d_out, int nslices,
                       double* h_xTx, void* stream);
 
+/* Mixed-precision T x (no counterpart in the reference): x rounded to FP16
+ * under a per-slice power-of-two scale, the three passes computing in FP32
+ * with FP16 intermediates and FP16 spectrum (the operator the mixed MBIR
+ * form applies to its gradient).  Needs an FFT length with compile-time
+ * passes.  Optional h_xTx as for tc_toeplitz_apply, here for the rounded x.
+ * Synchronises `stream`. */
+int tc_toeplitz_apply_mixed(tc_psf p, const float* d_x, float* d_out,
+                            int nslices, double* h_xTx, void* stream);
+
 /* tomocam::gradient2 (src/gradient2.cpp:36-100, src/tomocam.h:85-86):
  * d_g = T d_x - d_sinoT. */
 int tc_gradient2(tc_psf p, const float* d_x, const float* d_sinoT, float* d_g,
@@ -202,6 +211,24 @@ int tc_mbir2(const float* d_sino, const float* d_x0, float* d_recon,
              int nslices, int nproj, int nrays, const float* h_angles,
              float center, int num_iters, float sigma, float tol, float xtol,
              int refresh_every, int verbose, double* h_info, void* stream);
+
+/* Mixed-precision form of tc_mbir2 (no counterpart in the reference, whose
+ * Python path is FP32 throughout): the same pipeline and iteration, with
+ * the gradient stored in FP16 under a per-slice power-of-two scale, the
+ * Toeplitz intermediates and the PSF spectrum in FP16, FFT butterflies and
+ * NUFFT spreading in FP32, the iterate, T x and b in FP32, line-search dots
+ * in FP64, and T sol recomputed in FP32 every refresh_every accepted
+ * iterations (iterative refinement of the T y recurrence; 0 = never).
+ * lipschitz_iters > 0 takes L from that many power-iteration steps on T
+ * (FP64 dots) instead of max(T 1).  Needs an FFT length with compile-time
+ * passes (TC_ERR_INVALID otherwise).  Optional h_info[8] = tc_mbir2's six
+ * values, then the number of slice-attempts whose FP16 gradient saturated
+ * and the number of backtracks. */
+int tc_mbir2_mixed(const float* d_sino, const float* d_x0, float* d_recon,
+                   int nslices, int nproj, int nrays, const float* h_angles,
+                   float center, int num_iters, float sigma,
+                   int refresh_every, int lipschitz_iters, int verbose,
+                   double* h_info, void* stream);
 
 /* tomocam::mbir (src/mbir.cpp:38-119): the NUFFT normal-operator MBIR,
  * without a PSF.  Normalises by (sino - min) / (max - min), pads and

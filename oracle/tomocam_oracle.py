@@ -777,6 +777,20 @@ def mbir2_setup(sino: np.ndarray, angles: np.ndarray, center: float,
     return MBIRSetup(sino2, x0, sinoT, sino_norm, ps, L, step, nrays)
 
 
+def power_lipschitz(psf_arr: np.ndarray, n: int, iters: int) -> float:
+    """Largest eigenvalue of the Toeplitz operator by power iteration (FP64):
+    v0 = ones / n, then ``iters`` times w = T v, lam = v.w / v.v,
+    v = w / ||w||; returns the last lam.  (The mixed-precision form's option
+    in place of the reference's max(T 1), src/mbir2.cpp:101-112.)"""
+    v = np.full((1, n, n), 1.0 / n)
+    lam = 0.0
+    for _ in range(iters):
+        w = toeplitz(v, psf_arr)
+        lam = float(np.sum(v * w) / np.sum(v * v))
+        v = w / math.sqrt(float(np.sum(w * w)))
+    return lam
+
+
 def mbir2(sino: np.ndarray, angles: np.ndarray, center: float, num_iters: int,
           sigma: float, p: float = 1.2, exact: bool = False,
           log: Optional[Run2Log] = None, setup: Optional[MBIRSetup] = None):

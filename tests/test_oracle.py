@@ -338,3 +338,19 @@ def test_qggmrf_grad_free_is_gradient_of_value():
         # interior voxels: the reference (clamped) stencil
         gi = O.qggmrf_grad(x, sigma, 1.2)
         assert np.array_equal(gi[1:-1, 1:-1, 1:-1], g[1:-1, 1:-1, 1:-1])
+
+
+def test_power_lipschitz_is_largest_eigenvalue():
+    """Power iteration on T: its Rayleigh quotient never exceeds the largest
+    eigenvalue of the dense (symmetric) operator and converges to it."""
+    n, P = 12, 7
+    ang = np.arange(P) * np.pi / P
+    ps = O.psf(ang, n, exact=True)
+    eye = np.eye(n * n).reshape(n * n, n, n)
+    M = O.toeplitz(eye, ps).reshape(n * n, n * n)
+    assert np.abs(M - M.T).max() <= 1e-13 * np.abs(M).max()
+    lam = np.linalg.eigvalsh(0.5 * (M + M.T))
+    assert lam.min() >= -1e-12 * lam.max()   # A^H A / n^3: PSD
+    l20 = O.power_lipschitz(ps, n, 20)
+    assert 0.5 * lam.max() < l20 <= lam.max() * (1 + 1e-12)
+    assert abs(O.power_lipschitz(ps, n, 3000) - lam.max()) <= 1e-8 * lam.max()

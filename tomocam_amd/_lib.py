@@ -32,6 +32,7 @@ SIGNATURES = {
     "tc_psf_destroy": [vp],
     "tc_psf_info": [vp, c_int_p, c_int_p],
     "tc_toeplitz_apply": [vp, vp, vp, C.c_int, c_double_p, vp],
+    "tc_toeplitz_apply_mixed": [vp, vp, vp, C.c_int, c_double_p, vp],
     "tc_gradient2": [vp, vp, vp, vp, C.c_int, vp],
     "tc_function_value2": [vp, vp, vp, C.c_int, C.c_double, c_double_p, vp],
     "tc_add_total_var2": [vp, vp, C.c_int, C.c_int, C.c_int, C.c_float,
@@ -58,6 +59,9 @@ SIGNATURES = {
     "tc_mbir2": [vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, C.c_float,
                  C.c_int, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int,
                  c_double_p, vp],
+    "tc_mbir2_mixed": [vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, C.c_float,
+                       C.c_int, C.c_float, C.c_int, C.c_int, C.c_int,
+                       c_double_p, vp],
     "tc_mbir": [vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, C.c_float,
                 C.c_int, C.c_float, C.c_float, C.c_float, C.c_int, c_double_p,
                 vp],

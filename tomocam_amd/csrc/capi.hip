@@ -201,6 +201,34 @@ int tc_toeplitz_apply(tc_psf p, const float* d_x, float* d_out, int nslices,
   });
 }
 
+int tc_toeplitz_apply_mixed(tc_psf p, const float* d_x, float* d_out,
+                            int nslices, double* h_xTx, void* stream) {
+  return guarded([&] {
+    tc::Psf& P = get_psf(p);
+    TC_REQUIRE(d_x && d_out, "null argument");
+    TC_REQUIRE(nslices >= 1, "Toeplitz: need at least one slice");
+    TC_REQUIRE(P.has_fused(), "mixed precision needs a compile-time FFT "
+                              "plan; FFT length " + std::to_string(P.M()) +
+                              " has none");
+    hipStream_t s = as_stream(stream);
+    const long plane = (long)P.n() * P.n();
+    tc::DevBuf x16(2 * (size_t)nslices * plane);
+    tc::DevBuf gsc(sizeof(float) * (size_t)nslices);
+    tc::to_half_slices(d_x, x16.as<void>(), gsc.as<float>(), nslices, plane,
+                       s);
+    const int np = tc::toeplitz_num_partials(P, nslices);
+    tc::DevBuf parts(sizeof(double) * (np + 1));
+    P.apply_half(x16.as<void>(), gsc.as<float>(), d_out, nslices,
+                 h_xTx ? parts.as<double>() : nullptr, s);
+    if (h_xTx) {
+      tc::reduce_partials(parts.as<double>(), np, parts.as<double>() + np, s);
+      TC_HIP(hipMemcpyAsync(h_xTx, parts.as<double>() + np, sizeof(double),
+                            hipMemcpyDeviceToHost, s));
+    }
+    TC_HIP(hipStreamSynchronize(s));
+  });
+}
+
 int tc_gradient2(tc_psf p, const float* d_x, const float* d_sinoT, float* d_g,
                  int nslices, void* stream) {
   return guarded([&] {
@@ -332,6 +360,34 @@ int tc_mbir2(const float* d_sino, const float* d_x0, float* d_recon,
                                   nrays, h_angles, center, o,
                                   as_stream(stream));
     fill_info(r, h_info);
+  });
+}
+
+int tc_mbir2_mixed(const float* d_sino, const float* d_x0, float* d_recon,
+                   int nslices, int nproj, int nrays, const float* h_angles,
+                   float center, int num_iters, float sigma,
+                   int refresh_every, int lipschitz_iters, int verbose,
+                   double* h_info, void* stream) {
+  return guarded([&] {
+    TC_REQUIRE(d_sino && d_recon && h_angles, "null argument");
+    TC_REQUIRE(num_iters >= 0, "mbir: num_iters must be >= 0");
+    TC_REQUIRE(refresh_every >= 0, "mbir: refresh_every must be >= 0");
+    TC_REQUIRE(lipschitz_iters >= 0, "mbir: lipschitz_iters must be >= 0");
+    tc::Mbir2Options o;
+    o.num_iters = num_iters;
+    o.sigma = sigma;
+    o.refresh_every = refresh_every;
+    o.verbose = verbose;
+    o.mixed = true;
+    o.lipschitz_iters = lipschitz_iters;
+    tc::Mbir2Result r = tc::mbir2(d_sino, d_x0, d_recon, nslices, nproj,
+                                  nrays, h_angles, center, o,
+                                  as_stream(stream));
+    fill_info(r, h_info);
+    if (h_info) {
+      h_info[6] = r.stats.saturated;
+      h_info[7] = r.stats.backtracks;
+    }
   });
 }
 

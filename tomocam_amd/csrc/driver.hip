@@ -154,8 +154,28 @@ Mbir2Result mbir2(const float* sino, const float* x0, float* recon, int S,
 
   // L = max(Re(A^H A 1)/n^3) = max(T 1); the qGGMRF Hessian term never
   // reaches the host in the reference (src/mbir2.cpp:101-112,
-  // src/gpu/tv_hessian.cu:53-65).
-  {
+  // src/gpu/tv_hessian.cu:53-65).  Optionally the largest eigenvalue of T
+  // by power iteration instead (Rayleigh quotient after k steps).
+  if (o.lipschitz_iters > 0) {
+    const long nn = (long)n2 * n2;
+    DevBuf bv(sizeof(float) * nn), bw(sizeof(float) * nn);
+    float* v = bv.as<float>();
+    float* w = bw.as<float>();
+    hipLaunchKernelGGL(fill_kernel, dim3(cdiv(nn, 256)), dim3(256), 0, stream,
+                       v, nn, 1.0f / (float)n2);
+    TC_HIP(hipGetLastError());
+    double lam = 0.0;
+    for (int k = 0; k < o.lipschitz_iters; ++k) {
+      psf.apply(v, nullptr, w, 1, 0, nullptr, stream);
+      const double vv = dot(v, v, nn, stream), vw = dot(v, w, nn, stream);
+      const double ww = dot(w, w, nn, stream);
+      TC_REQUIRE(vv > 0.0 && ww > 0.0 && std::isfinite(ww),
+                 "mbir2: power iteration on T degenerated");
+      lam = vw / vv;
+      affine(w, v, nn, 0.f, (float)(1.0 / std::sqrt(ww)), stream);
+    }
+    res.lipschitz = lam;
+  } else {
     const long nn = (long)n2 * n2;
     DevBuf ones(sizeof(float) * nn), t1(sizeof(float) * nn);
     hipLaunchKernelGGL(fill_kernel, dim3(cdiv(nn, 256)), dim3(256), 0, stream,
@@ -179,6 +199,7 @@ Mbir2Result mbir2(const float* sino, const float* x0, float* recon, int S,
   ro.sino_norm = res.sino_norm;
   ro.refresh_every = o.refresh_every;
   ro.verbose = o.verbose;
+  ro.mixed = o.mixed;
   DevBuf bsol(sizeof(float) * (size_t)S * n2 * n2);
   res.stats = run2_device(psf, bx0p.as<float>(), bb.as<float>(),
                           bsol.as<float>(), S, ro, stream);

@@ -29,6 +29,11 @@ void add_qggmrf_grad_free(const float* x, float* g, int S, int R, int C,
 double qggmrf_value_free(const float* x, int S, int R, int C, float sigma,
                          float p, hipStream_t stream);
 
+// x [S][plane] -> FP16 x * gsc[s] with gsc[s] the power of two that maps
+// max |x[s]| into (2^11, 2^12] (1 for an all-zero slice).
+void to_half_slices(const float* x, void* x16, float* gsc, int S, long plane,
+                    hipStream_t stream);
+
 struct Run2Options {
   int num_iters = 100;
   double step = 1.0;
@@ -36,6 +41,12 @@ struct Run2Options {
   float p = 1.2f;
   double sino_norm = 0.0;  // ||y||^2 of the (normalised, padded) sinogram
   int refresh_every = 10;  // recompute T(sol), T(x) every k accepted steps
+  // Mixed precision: the gradient is stored in FP16 with a per-slice
+  // power-of-two scale and T g runs the FP16-storage Toeplitz passes; the
+  // state (sol, x, T sol, T x, b) stays FP32 and the dots FP64.  The
+  // refresh above recomputes T sol in FP32, bounding the drift that the
+  // FP16 T g leaves in the T y recurrence.
+  bool mixed = false;
   int verbose = 0;         // print the reference's per-iteration line
   // optional per-iteration record: (iter, error, x-err, step, backtracks)
   std::function<void(int, double, double, double, int)> on_iter;
@@ -48,6 +59,7 @@ struct Run2Stats {
   int toeplitz_applies = 0;
   double last_error = 0.0;
   double last_xerr = 0.0;
+  int saturated = 0;  // mixed precision: slice-attempts whose FP16 g clipped
 };
 
 // Optimizer::run2 (src/optimize.h:96-154) on device-resident state.

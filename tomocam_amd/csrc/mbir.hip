@@ -1,4 +1,6 @@
 // qGGMRF stencil, fused MBIR gradient/commit kernels and the run2 driver.
+#include <hip/hip_fp16.h>
+
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -46,6 +48,11 @@ struct QArgs {
   const float* b;
   float* g;
   double* partials;    // NDOT per workgroup (MBIR mode)
+  // FP16 gradient (mixed precision): g16 = g * gsc[slice], and the largest
+  // |g| of each wave and plane into gmaxp
+  __half* g16;
+  const float* gsc;
+  float* gmaxp;
   int S, R, C;
   int zchunk;
   float beta;
@@ -194,7 +201,9 @@ __device__ __forceinline__ void qload(float (&v)[QJ], const QArgs& a, int z,
 // MODE 0: g += R'(x).  MODE 1 (MBIR): y = sol + beta (sol - xprev),
 // g = Tsol + beta (Tsol - Tx) - b + R'(y), plus the line-search dots.
 // HAS_X = false means xprev == sol and Tx == Tsol (y = sol, d = 0).
-template <int MODE, bool HAS_X>
+// G16 (MODE 1): g is stored in FP16 times the plane's scale (saturating at
+// the largest finite value), and the dots use the stored value.
+template <int MODE, bool HAS_X, bool G16 = false>
 __global__ void __launch_bounds__(256) qggmrf_kernel(QArgs a) {
   __shared__ double red[16];
   const int lane = threadIdx.x & 63;
@@ -244,9 +253,16 @@ __global__ void __launch_bounds__(256) qggmrf_kernel(QArgs a) {
     qplane(vc, gc, a);
     qcross<true>(vc, vn, gc, gn, a);
 
+    float gm = 0.f;  // largest |g| of the wave in this plane (G16)
     if (xout) {
       const long zo = (long)z * plane;
       float* gp = a.g + zo;
+      char* g16p = reinterpret_cast<char*>(a.g16 + (G16 ? zo : 0));
+      float gs = 1.f, gsi = 1.f;
+      if constexpr (G16) {
+        gs = a.gsc[z];
+        gsi = 1.0f / gs;
+      }
       const float* bp = a.b + zo;
       const float* tsp = a.Tsol + zo;
       const float* txp = a.Tx + zo;
@@ -264,8 +280,16 @@ __global__ void __launch_bounds__(256) qggmrf_kernel(QArgs a) {
             float tyv = at_byte(tsp, o);
             if constexpr (HAS_X)
               tyv = fmaf(a.beta, tyv - at_byte(txp, o), tyv);
-            const float gv = (tyv - bb) + gc[j];
-            at_byte(gp, o) = gv;
+            float gv = (tyv - bb) + gc[j];
+            if constexpr (G16) {
+              gm = fmaxf(gm, fabsf(gv));
+              const __half h =
+                  __float2half_rn(fminf(fmaxf(gv * gs, -65504.f), 65504.f));
+              *reinterpret_cast<__half*>(g16p + (o >> 1)) = h;
+              gv = __half2float(h) * gsi;
+            } else {
+              at_byte(gp, o) = gv;
+            }
             acc[0] += (double)v * tyv;
             acc[1] += (double)v * bb;
             acc[2] += (double)gv * (tyv - bb);
@@ -278,6 +302,11 @@ __global__ void __launch_bounds__(256) qggmrf_kernel(QArgs a) {
           }
         }
       }
+    }
+    if constexpr (G16) {
+      for (int o = 32; o > 0; o >>= 1) gm = fmaxf(gm, __shfl_xor(gm, o));
+      if (lane == 0)
+        a.gmaxp[((long)bl * QWAVES + wave) * a.zchunk + (z - z0)] = gm;
     }
 #pragma unroll
     for (int j = 0; j < QJ; ++j) {
@@ -521,6 +550,78 @@ __global__ void reduce_strided_kernel(const double* __restrict__ p, long count,
   if (threadIdx.x == 0) out[k] = tot;
 }
 
+// ---------------------------------------------------------------------------
+// per-slice scales of the FP16 gradient (mixed precision)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float block_max256(float m, float* red) {
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  return fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+
+// power of two mapping m into (2^11, 2^12]; `keep` when m is 0 or not finite
+__device__ __forceinline__ float half_scale_for(float m, float keep) {
+  if (!(m > 0.f && m <= 3.0e38f)) return keep;
+  int ex = 0;
+  (void)frexpf(m, &ex);
+  return ldexpf(1.f, 12 - max(ex, -100));
+}
+
+// gsc[s] from max |b[s]| (the first gradient is about -b: x0 is small)
+__global__ void __launch_bounds__(256)
+gscale_init_kernel(const float* __restrict__ b, long plane, float* gsc) {
+  __shared__ float red[4];
+  const float* p = b + (long)blockIdx.x * plane;
+  float m = 0.f;
+  for (long i = threadIdx.x; i < plane; i += 256) m = fmaxf(m, fabsf(p[i]));
+  m = block_max256(m, red);
+  if (threadIdx.x == 0) gsc[blockIdx.x] = half_scale_for(m, 1.f);
+}
+
+// next[s] from the largest |g| of slice s this attempt (the stencil's
+// per-wave maxima); counts slices whose stored values saturated
+__global__ void __launch_bounds__(256)
+gscale_update_kernel(const float* __restrict__ gmaxp, int ntiles, int zchunk,
+                     const float* __restrict__ cur, float* next,
+                     int* saturated) {
+  __shared__ float red[4];
+  const int s = blockIdx.x;
+  const int zc = s / zchunk, off = s - zc * zchunk;
+  const long nw = (long)ntiles * QWAVES;
+  const long base = (long)zc * nw;
+  float m = 0.f;
+  for (long i = threadIdx.x; i < nw; i += 256)
+    m = fmaxf(m, gmaxp[(base + i) * zchunk + off]);
+  m = block_max256(m, red);
+  if (threadIdx.x == 0) {
+    const float c = cur[s];
+    next[s] = half_scale_for(m, c);
+    if (!(m * c <= 65504.f)) atomicAdd(saturated, 1);
+  }
+}
+
+__global__ void to_half_slices_kernel(const float* __restrict__ x,
+                                      __half* out, long plane, long n,
+                                      const float* __restrict__ gsc) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    out[i] = __float2half_rn(
+        fminf(fmaxf(x[i] * gsc[i / plane], -65504.f), 65504.f));
+}
+
+void to_half_slices(const float* x, void* x16, float* gsc, int S, long plane,
+                    hipStream_t stream) {
+  hipLaunchKernelGGL(gscale_init_kernel, dim3(S), dim3(256), 0, stream, x,
+                     plane, gsc);
+  TC_HIP(hipGetLastError());
+  const long n = (long)S * plane;
+  hipLaunchKernelGGL(to_half_slices_kernel, dim3(cdiv(n, 256)), dim3(256), 0,
+                     stream, x, static_cast<__half*>(x16), plane, n, gsc);
+  TC_HIP(hipGetLastError());
+}
+
 // Optimizer::run2 (src/optimize.h:96-154), restated on device-resident state.
 // Per attempt the reference evaluates gradient_(y), error_(sol), error_(y)
 // (and error_(sol) again after acceptance): four Toeplitz applications.  By
@@ -561,6 +662,28 @@ Run2Stats run2_device(Psf& psf, const float* x0, const float* b, float* out,
   double* qparts = parts.as<double>();
   double* tparts = qparts + (size_t)qblk * NDOT;
   qa.partials = qparts;
+  // mixed precision: FP16 gradient, per-slice scales (this attempt's and
+  // the next one's), per-wave maxima, saturation counter
+  DevBuf bg16, bgsc, bgmax, bsat;
+  float *gsc = nullptr, *gsc_next = nullptr;
+  if (opt.mixed) {
+    TC_REQUIRE(psf.has_fused(),
+               "mixed precision needs a compile-time FFT plan; FFT length " +
+                   std::to_string(psf.M()) + " has none");
+    bg16.alloc(sizeof(__half) * (size_t)N);
+    bgsc.alloc(sizeof(float) * 2 * (size_t)S);
+    bgmax.alloc(sizeof(float) * (size_t)qblk * QWAVES * qa.zchunk);
+    bsat.alloc(sizeof(int));
+    TC_HIP(hipMemsetAsync(bsat.as<void>(), 0, sizeof(int), stream));
+    gsc = bgsc.as<float>();
+    gsc_next = gsc + S;
+    qa.g16 = bg16.as<__half>();
+    qa.gmaxp = bgmax.as<float>();
+    hipLaunchKernelGGL(gscale_init_kernel, dim3(S), dim3(256), 0, stream, b,
+                       (long)n * n, gsc);
+    TC_HIP(hipGetLastError());
+  }
+  const int ntiles_q = (int)(qblk / cdiv(S, qa.zchunk));
   DevBuf dots_dev(sizeof(double) * 8);
   double* dots_h = nullptr;
   TC_HIP(hipHostMalloc((void**)&dots_h, sizeof(double) * 8, hipHostMallocDefault));
@@ -599,16 +722,32 @@ Run2Stats run2_device(Psf& psf, const float* x0, const float* b, float* out,
       qa.Tsol = Tsol;
       qa.Tx = has_x ? Tx : Tsol;
       qa.beta = has_x ? (float)beta : 0.0f;
+      qa.gsc = gsc;
       ktimer_begin(KC_MBIR_GRADIENT, stream);
-      if (has_x)
+      if (opt.mixed) {
+        if (has_x)
+          hipLaunchKernelGGL((qggmrf_kernel<1, true, true>), dim3(qblk),
+                             dim3(256), 0, stream, qa);
+        else
+          hipLaunchKernelGGL((qggmrf_kernel<1, false, true>), dim3(qblk),
+                             dim3(256), 0, stream, qa);
+      } else if (has_x) {
         hipLaunchKernelGGL((qggmrf_kernel<1, true>), dim3(qblk), dim3(256), 0,
                            stream, qa);
-      else
+      } else {
         hipLaunchKernelGGL((qggmrf_kernel<1, false>), dim3(qblk), dim3(256), 0,
                            stream, qa);
+      }
       TC_HIP(hipGetLastError());
       // reads sol, T sol, b (and x, T x unless aliased), writes g
-      ktimer_end(KC_MBIR_GRADIENT, stream, (has_x ? 6.0 : 4.0) * vb);
+      ktimer_end(KC_MBIR_GRADIENT, stream,
+                 (has_x ? 5.0 : 3.0) * vb + (opt.mixed ? 0.5 : 1.0) * vb);
+      if (opt.mixed) {
+        hipLaunchKernelGGL(gscale_update_kernel, dim3(S), dim3(256), 0, stream,
+                           qa.gmaxp, ntiles_q, qa.zchunk, gsc, gsc_next,
+                           bsat.as<int>());
+        TC_HIP(hipGetLastError());
+      }
       // The step is committed inside the Toeplitz passes where the FFT size
       // allows: cand = y - s g and T cand = T y - s T g are written before
       // the accept test, which then only decides the buffers' roles.  In
@@ -619,7 +758,10 @@ Run2Stats run2_device(Psf& psf, const float* x0, const float* b, float* out,
                                has_x ? Tx : nullptr, fdst, fTdst,
                                has_x ? (float)beta : 0.0f, (float)step};
       const bool fused =
-          allow_fused && psf.apply_commit(g, sc, S, tparts, stream);
+          opt.mixed
+              ? psf.apply_commit_half(qa.g16, gsc, sc, S, tparts, stream)
+              : allow_fused && psf.apply_commit(g, sc, S, tparts, stream);
+      if (opt.mixed) std::swap(gsc, gsc_next);
       if (!fused) psf.apply(g, nullptr, Tg, S, 0, tparts, stream);
       ++st.toeplitz_applies;
       hipLaunchKernelGGL(reduce_strided_kernel, dim3(NDOT), dim3(1024), 0,
@@ -697,6 +839,9 @@ Run2Stats run2_device(Psf& psf, const float* x0, const float* b, float* out,
                    std::sqrt(xerr));
   }
   TC_HIP(hipMemcpyAsync(out, sol, vb, hipMemcpyDeviceToDevice, stream));
+  if (opt.mixed)
+    TC_HIP(hipMemcpyAsync(&st.saturated, bsat.as<int>(), sizeof(int),
+                          hipMemcpyDeviceToHost, stream));
   TC_HIP(hipStreamSynchronize(stream));
   return st;
 }

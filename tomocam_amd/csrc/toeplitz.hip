@@ -1,5 +1,7 @@
 // Toeplitz normal operator: PSF spectrum set-up and the three-pass apply.
 // Reference: src/toeplitz.h:51-159 (PointSpreadFunction ctor / convolve).
+#include <hip/hip_fp16.h>
+
 #include <cmath>
 
 #include "fft_device.h"
@@ -260,6 +262,100 @@ bool toeplitz_apply_ct(int M, const float* x, const float* b, float* out,
                        int nslices, int mode, double* partials,
                        hipStream_t stream, double img_b, double spec_b,
                        const Psf::StepCommit* cm);
+
+bool toeplitz_apply_ct_half(int M, const void* x16, float* out, void* wt,
+                            const void* spec16, int n, int H, int rp,
+                            int nslices, double* partials, hipStream_t stream,
+                            double img_b, double spec_b,
+                            const Psf::StepCommit* sc,
+                            const ToeplitzHalf& half);
+int toeplitz_half_group(int nslices, int H, int rp);
+
+// max |v[i]| over n floats into out[0] (one workgroup)
+__global__ void absmax_kernel(const float* __restrict__ v, long n,
+                              float* out) {
+  __shared__ float red[16];
+  float m = 0.f;
+  for (long i = threadIdx.x; i < n; i += blockDim.x) m = fmaxf(m, fabsf(v[i]));
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, red[w]);
+    out[0] = m;
+  }
+}
+
+__global__ void to_half_kernel(const float* __restrict__ v, __half* out,
+                               long n, float scale) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = __float2half_rn(v[i] * scale);
+}
+
+bool Psf::run_half(const void* g16, const float* gsc, float* out,
+                   int nslices, double* dot_partials, const StepCommit* c,
+                   hipStream_t stream) {
+  TC_REQUIRE(nslices >= 1, "Toeplitz: need at least one slice");
+  TC_REQUIRE(g16 && gsc, "Toeplitz: null FP16 input or scales");
+  if (toeplitz_ct_lpw(M_) == 0) return false;
+  const long nspec = (long)M_ * H_;
+  if (spec16_.bytes() == 0) {
+    // FP16 copy of the spectrum, its largest magnitude mapped into
+    // (2^13, 2^14]
+    DevBuf mx(sizeof(float));
+    hipLaunchKernelGGL(absmax_kernel, dim3(1), dim3(1024), 0, stream,
+                       spectrum(), nspec, mx.as<float>());
+    TC_HIP(hipGetLastError());
+    float m = 0.f;
+    TC_HIP(hipMemcpyAsync(&m, mx.as<float>(), sizeof(float),
+                          hipMemcpyDeviceToHost, stream));
+    TC_HIP(hipStreamSynchronize(stream));
+    TC_REQUIRE(m > 0.f && std::isfinite(m), "Toeplitz: degenerate spectrum");
+    int ex = 0;
+    (void)std::frexp(m, &ex);
+    spec16_exp_ = 14 - ex;
+    spec16_.alloc(sizeof(__half) * (size_t)nspec);
+    hipLaunchKernelGGL(to_half_kernel, dim3(cdiv(nspec, 256)), dim3(256), 0,
+                       stream, spectrum(), spec16_.as<__half>(), nspec,
+                       std::ldexp(1.0f, spec16_exp_));
+    TC_HIP(hipGetLastError());
+  }
+  reserve(nslices);
+  const int group = toeplitz_half_group(nslices, H_, rp_);
+  if ((size_t)group > cinv_slices_) {
+    cinv_.alloc(sizeof(float) * (size_t)group * H_);
+    cinv_slices_ = group;
+  }
+  int nb = 0;
+  (void)std::frexp((double)n_, &nb);  // 2^(nb-1) <= n < 2^nb
+  ToeplitzHalf h;
+  h.gsc = gsc;
+  h.cinv = cinv_.as<float>();
+  h.wsc = std::ldexp(1.0f, -(nb + 1));  // n * 2^16 * wsc < 2^15
+  h.cbase = std::ldexp(1.0f, nb + 1 - spec16_exp_);
+  h.pconst = std::ldexp(1.0f, 2 * (nb + 1) - spec16_exp_);
+  const double img_b = 4.0 * nslices * (double)n_ * n_;
+  const double spec_b = 8.0 * nslices * (double)n_ * H_;
+  return toeplitz_apply_ct_half(M_, g16, c ? c->Tdst : out, work_.as<void>(),
+                                spec16_.as<void>(), n_, H_, rp_, nslices,
+                                dot_partials, stream, img_b, spec_b, c, h);
+}
+
+bool Psf::has_fused() const { return toeplitz_ct_lpw(M_) > 0; }
+
+bool Psf::apply_half(const void* g16, const float* gsc, float* out,
+                     int nslices, double* dot_partials, hipStream_t stream) {
+  TC_REQUIRE(out != nullptr, "Toeplitz: null output");
+  return run_half(g16, gsc, out, nslices, dot_partials, nullptr, stream);
+}
+
+bool Psf::apply_commit_half(const void* g16, const float* gsc,
+                            const StepCommit& c, int nslices,
+                            double* dot_partials, hipStream_t stream) {
+  TC_REQUIRE((c.xp == nullptr) == (c.Tx == nullptr),
+             "Toeplitz: xp and Tx must both be given or both be null");
+  return run_half(g16, gsc, nullptr, nslices, dot_partials, &c, stream);
+}
 
 bool Psf::apply_commit(const float* g, const StepCommit& c, int nslices,
                        double* dot_partials, hipStream_t stream) {

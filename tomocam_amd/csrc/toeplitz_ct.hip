@@ -3,6 +3,8 @@
 #include <algorithm>
 #include <cstdlib>
 
+#include <hip/hip_fp16.h>
+
 #include "fft_ct.h"
 #include "lds_launch.h"
 #include "toeplitz.h"
@@ -81,11 +83,20 @@ struct CommitPtrs {
 //   (+ dst = y - s x when committing a step: x is the gradient there)
 // ---------------------------------------------------------------------------
 
-template <class P, int CM>
+// FP16 storage helpers of the mixed-precision passes
+__device__ __forceinline__ unsigned pack_half2(float a, float b) {
+  const __half2 h = __floats2half2_rn(a, b);
+  return *reinterpret_cast<const unsigned*>(&h);
+}
+__device__ __forceinline__ float2 unpack_half2(unsigned u) {
+  return __half22float2(*reinterpret_cast<const __half2*>(&u));
+}
+
+template <class P, int CM, bool H16>
 __global__ void __launch_bounds__(P::NT)
-ct_rows_fwd(const float* __restrict__ x, float2* __restrict__ wt,
+ct_rows_fwd(const void* __restrict__ xv, void* __restrict__ wtv,
             const float2* __restrict__ tw, int n, int H, int rp, int nbps,
-            CommitPtrs cp) {
+            CommitPtrs cp, ToeplitzHalf ha) {
   extern __shared__ float2 sm[];
   constexpr int L = P::L, E = P::E, T = P::T, LPW = P::LPW;
   using F = Xform<L, E, -1, typename P::Fwd>;
@@ -96,8 +107,6 @@ ct_rows_fwd(const float* __restrict__ x, float2* __restrict__ wt,
   const int s = b / nbps;
   const int r0 = (b - s * nbps) * 2 * LPW;
   const int ra = r0 + 2 * line;
-  const float* xs = x + (long)s * n * n;
-  const float* rowa = xs + (long)ra * n;
   const bool hasa = ra < n, hasb = ra + 1 < n;
   // All loads first, then the commit's stores: interleaved, the compiler
   // cannot prove the stores miss later loads (dst may alias sol) and would
@@ -105,6 +114,12 @@ ct_rows_fwd(const float* __restrict__ x, float2* __restrict__ wt,
   float2 v[E];
   float2 y[CM != 0 ? E : 1];
   const long oa = (long)s * n * n + (long)ra * n;
+  auto ld = [&](long o) -> float {
+    if constexpr (H16)
+      return __half2float(reinterpret_cast<const __half*>(xv)[o]);
+    else
+      return reinterpret_cast<const float*>(xv)[o];
+  };
 #pragma unroll
   for (int i = 0; i < E / R1; ++i)
 #pragma unroll
@@ -113,8 +128,8 @@ ct_rows_fwd(const float* __restrict__ x, float2* __restrict__ wt,
       const int e = i * R1 + r;
       float re = 0.f, im = 0.f;
       if (c < n) {
-        if (hasa) re = rowa[c];
-        if (hasb) im = rowa[n + c];
+        if (hasa) re = ld(oa + c);
+        if (hasb) im = ld(oa + n + c);
         if constexpr (CM != 0) {
           float ya = 0.f, yb = 0.f;
           if (hasa) {
@@ -132,6 +147,8 @@ ct_rows_fwd(const float* __restrict__ x, float2* __restrict__ wt,
       v[e] = make_float2(re, im);
     }
   if constexpr (CM != 0) {
+    // the step in the input's true units (FP16 input: x = g16 / gsc)
+    const float se = H16 ? cp.s / ha.gsc[s] : cp.s;
 #pragma unroll
     for (int i = 0; i < E / R1; ++i)
 #pragma unroll
@@ -139,8 +156,8 @@ ct_rows_fwd(const float* __restrict__ x, float2* __restrict__ wt,
         const int c = t + i * T + r * (L / R1);
         const int e = i * R1 + r;
         if (c < n) {
-          if (hasa) cp.dst[oa + c] = fmaf(-cp.s, v[e].x, y[e].x);
-          if (hasb) cp.dst[oa + n + c] = fmaf(-cp.s, v[e].y, y[e].y);
+          if (hasa) cp.dst[oa + c] = fmaf(-se, v[e].x, y[e].x);
+          if (hasb) cp.dst[oa + n + c] = fmaf(-se, v[e].y, y[e].y);
         }
       }
   }
@@ -151,6 +168,7 @@ ct_rows_fwd(const float* __restrict__ x, float2* __restrict__ wt,
 #pragma unroll
     for (int r = 0; r < RS; ++r) ln[t + i * T + r * (L / RS)] = v[i * RS + r];
   __syncthreads();
+  const float hw = H16 ? 0.5f * ha.wsc : 0.5f;
   for (int q = threadIdx.x; q < LPW * H; q += P::NT) {
     const int k = q / LPW;
     const int l = q - k * LPW;
@@ -159,8 +177,17 @@ ct_rows_fwd(const float* __restrict__ x, float2* __restrict__ wt,
       const float2 zk = sm[l * L + k];
       const float2 zm = cconj(sm[l * L + (k == 0 ? 0 : L - k)]);
       const float2 sp = cadd(zk, zm), df = csub(zk, zm);
-      *reinterpret_cast<float4*>(wt + wt_at(s, k, rr, H, rp)) =
-          make_float4(0.5f * sp.x, 0.5f * sp.y, 0.5f * df.y, -0.5f * df.x);
+      if constexpr (H16) {
+        // |value| <= n * 2^16 * wsc <= 2^15: no overflow
+        *reinterpret_cast<uint2*>(reinterpret_cast<unsigned*>(wtv) +
+                                  wt_at(s, k, rr, H, rp)) =
+            make_uint2(pack_half2(hw * sp.x, hw * sp.y),
+                       pack_half2(hw * df.y, -hw * df.x));
+      } else {
+        *reinterpret_cast<float4*>(reinterpret_cast<float2*>(wtv) +
+                                   wt_at(s, k, rr, H, rp)) =
+            make_float4(hw * sp.x, hw * sp.y, hw * df.y, -hw * df.x);
+      }
     }
   }
 }
@@ -170,14 +197,15 @@ ct_rows_fwd(const float* __restrict__ x, float2* __restrict__ wt,
 // without leaving registers between the two transforms
 // ---------------------------------------------------------------------------
 
-template <class P>
+template <class P, bool H16>
 __global__ void __launch_bounds__(P::NTC)
-ct_cols(float2* __restrict__ wt, const float* __restrict__ spec,
+ct_cols(void* __restrict__ wtv, const void* __restrict__ specv,
         const float2* __restrict__ tw, int n, int H, int rp, int nbps,
-        double* partials) {
+        double* partials, ToeplitzHalf ha) {
   extern __shared__ float2 sm[];
   __shared__ double red[16];
   constexpr int L = P::L, E = P::E, T = P::T, LPW = P::LPWC;
+  __shared__ unsigned cmax[H16 ? LPW : 1];
   using F = Xform<L, E, -1, typename P::Fwd>;
   using I = Xform<L, E, 1, typename P::Inv>;
   constexpr int R1 = F::first, RS = F::last;
@@ -193,24 +221,40 @@ ct_cols(float2* __restrict__ wt, const float* __restrict__ spec,
   const int k1 = kg * LPW + line;
   const bool active = k1 < H;
   const int kc = active ? k1 : 0;
+  if constexpr (H16) {
+    if (threadIdx.x < LPW) cmax[threadIdx.x] = 0u;
+  }
   float2 v[E];
 #pragma unroll
   for (int i = 0; i < E / R1; ++i)
 #pragma unroll
     for (int r = 0; r < R1; ++r) {
       const int idx = t + i * T + r * (L / R1);
-      v[i * R1 + r] = (active && idx < n) ? wt[wt_at(s, kc, idx, H, rp)]
-                                          : make_float2(0.f, 0.f);
+      float2 w = make_float2(0.f, 0.f);
+      if (active && idx < n) {
+        const long o = wt_at(s, kc, idx, H, rp);
+        if constexpr (H16)
+          w = unpack_half2(reinterpret_cast<const unsigned*>(wtv)[o]);
+        else
+          w = reinterpret_cast<const float2*>(wtv)[o];
+      }
+      v[i * R1 + r] = w;
     }
   // the spectrum values this thread multiplies by, loaded up front so
   // their latency overlaps the forward transform
   float hs[E];
   {
-    const float* sp = spec + (long)kc * L;
+    const long so = (long)kc * L;
 #pragma unroll
     for (int i = 0; i < E / RS; ++i)
 #pragma unroll
-      for (int r = 0; r < RS; ++r) hs[i * RS + r] = sp[t + i * T + r * (L / RS)];
+      for (int r = 0; r < RS; ++r) {
+        const long o = so + t + i * T + r * (L / RS);
+        if constexpr (H16)
+          hs[i * RS + r] = __half2float(reinterpret_cast<const __half*>(specv)[o]);
+        else
+          hs[i * RS + r] = reinterpret_cast<const float*>(specv)[o];
+      }
   }
   float2* ln = sm + line * L;
   F::run(v, ln, t, active, tw);
@@ -231,16 +275,54 @@ ct_cols(float2* __restrict__ wt, const float* __restrict__ spec,
   }
   // the inverse's first stage needs no exchange (same element set)
   I::run(v, ln, t, active, tw);
-  if (active) {
+  if constexpr (H16) {
+    // per-column power-of-two scale: the column's largest kept value maps
+    // below 2^14
+    float m = 0.f;
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < E / R1; ++i)
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+          const int idx = t + i * T + r * (L / R1);
+          if (idx < n)
+            m = fmaxf(m, fmaxf(fabsf(v[i * R1 + r].x), fabsf(v[i * R1 + r].y)));
+        }
+      atomicMax(&cmax[line], __float_as_uint(m));
+    }
+    __syncthreads();
+    if (active) {
+      const float mm = __uint_as_float(cmax[line]);
+      int ex = 14;
+      if (mm > 0.f && mm <= 3.0e38f) (void)frexpf(mm, &ex);
+      const float kap = ldexpf(1.f, 14 - ex);
+#pragma unroll
+      for (int i = 0; i < E / R1; ++i)
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+          const int idx = t + i * T + r * (L / R1);
+          if (idx < n)
+            reinterpret_cast<unsigned*>(wtv)[wt_at(s, kc, idx, H, rp)] =
+                pack_half2(kap * v[i * R1 + r].x, kap * v[i * R1 + r].y);
+        }
+      if (t == 0) ha.cinv[(long)s * H + k1] = ldexpf(ha.cbase, ex - 14);
+    }
+  } else if (active) {
 #pragma unroll
     for (int i = 0; i < E / R1; ++i)
 #pragma unroll
       for (int r = 0; r < R1; ++r) {
         const int idx = t + i * T + r * (L / R1);
-        if (idx < n) wt[wt_at(s, kc, idx, H, rp)] = v[i * R1 + r];
+        if (idx < n)
+          reinterpret_cast<float2*>(wtv)[wt_at(s, kc, idx, H, rp)] =
+              v[i * R1 + r];
       }
   }
   if (partials) {
+    if constexpr (H16) {
+      const float gs = ha.gsc[s];
+      acc *= (double)ha.pconst / ((double)gs * gs);
+    }
     double tot = block_sum(acc, red);
     if (threadIdx.x == 0) partials[b] = tot;
   }
@@ -252,11 +334,11 @@ ct_cols(float2* __restrict__ wt, const float* __restrict__ spec,
 //   2 / 3: out = Ty - s T x with Ty = Tsol (2) or Tsol + beta (Tsol - Tx) (3)
 // ---------------------------------------------------------------------------
 
-template <class P, int MODE>
+template <class P, int MODE, bool H16>
 __global__ void __launch_bounds__(P::NT)
-ct_rows_inv(const float2* __restrict__ wt, const float* bvec, float* out,
+ct_rows_inv(const void* __restrict__ wtv, const float* bvec, float* out,
             const float2* __restrict__ tw, int n, int H, int rp, int nbps,
-            CommitPtrs cp) {
+            CommitPtrs cp, ToeplitzHalf ha) {
   extern __shared__ float2 sm[];
   constexpr int L = P::L, E = P::E, T = P::T, LPW = P::LPW;
   using I = Xform<L, E, 1, typename P::Inv>;
@@ -270,14 +352,27 @@ ct_rows_inv(const float2* __restrict__ wt, const float* bvec, float* out,
   constexpr int HC = L / 2 + 1;
   constexpr int NQ = (LPW * HC + P::NT - 1) / P::NT;
   float4 w4[NQ];
+  float cs[H16 ? NQ : 1];
 #pragma unroll
   for (int j = 0; j < NQ; ++j) {
     const int q = threadIdx.x + j * P::NT;
     const int k = q / LPW;
     const int rr = r0 + 2 * (q - k * LPW);
     w4[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (q < LPW * H && rr < rp)
-      w4[j] = *reinterpret_cast<const float4*>(wt + wt_at(s, k, rr, H, rp));
+    if constexpr (H16) cs[j] = 0.f;
+    if (q < LPW * H && rr < rp) {
+      const long o = wt_at(s, k, rr, H, rp);
+      if constexpr (H16) {
+        const uint2 u = *reinterpret_cast<const uint2*>(
+            reinterpret_cast<const unsigned*>(wtv) + o);
+        const float2 a = unpack_half2(u.x), c = unpack_half2(u.y);
+        w4[j] = make_float4(a.x, a.y, c.x, c.y);
+        cs[j] = ha.cinv[(long)s * H + k];
+      } else {
+        w4[j] = *reinterpret_cast<const float4*>(
+            reinterpret_cast<const float2*>(wtv) + o);
+      }
+    }
   }
 #pragma unroll
   for (int j = 0; j < NQ; ++j) {
@@ -286,6 +381,12 @@ ct_rows_inv(const float2* __restrict__ wt, const float* bvec, float* out,
       const int k = q / LPW;
       const int l = q - k * LPW;
       float4 w = w4[j];
+      if constexpr (H16) {
+        w.x *= cs[j];
+        w.y *= cs[j];
+        w.z *= cs[j];
+        w.w *= cs[j];
+      }
       if (k == 0 || 2 * k == L) {
         w.y = 0.f;
         w.w = 0.f;
@@ -307,6 +408,8 @@ ct_rows_inv(const float2* __restrict__ wt, const float* bvec, float* out,
   const int ra = r0 + 2 * line;
   const long oa = (long)s * n * n + (long)ra * n;
   const bool hasa = ra < n, hasb = ra + 1 < n;
+  // FP16 input: the transform is in units of gsc times the true values
+  const float gi = H16 ? 1.0f / ha.gsc[s] : 1.0f;
   // epilogue operands (b, or T sol and T x) all loaded before any store,
   // which the compiler could not otherwise move past (out may alias Tsol).
   // (Issuing them before the transform measured slower: more registers.)
@@ -332,6 +435,7 @@ ct_rows_inv(const float2* __restrict__ wt, const float* bvec, float* out,
         if constexpr (MODE == 3) e1[e] = p1;
       }
   }
+  const float se = (MODE >= 2 ? cp.s : 1.0f) * gi;
 #pragma unroll
   for (int i = 0; i < E / RS; ++i)
 #pragma unroll
@@ -341,17 +445,22 @@ ct_rows_inv(const float2* __restrict__ wt, const float* bvec, float* out,
       if (c < n) {
         const float2 q = v[e];
         float2 o = q;
-        if constexpr (MODE == 1) {
-          o.x = q.x - e0[e].x;
-          o.y = q.y - e0[e].y;
-        } else if constexpr (MODE >= 2) {
+        if constexpr (MODE == 0) {
+          if constexpr (H16) {
+            o.x = q.x * gi;
+            o.y = q.y * gi;
+          }
+        } else if constexpr (MODE == 1) {
+          o.x = fmaf(q.x, gi, -e0[e].x);
+          o.y = fmaf(q.y, gi, -e0[e].y);
+        } else {
           float2 ty = e0[e];
           if constexpr (MODE == 3) {
             ty.x = fmaf(cp.beta, ty.x - e1[e].x, ty.x);
             ty.y = fmaf(cp.beta, ty.y - e1[e].y, ty.y);
           }
-          o.x = fmaf(-cp.s, q.x, ty.x);
-          o.y = fmaf(-cp.s, q.y, ty.y);
+          o.x = fmaf(-se, q.x, ty.x);
+          o.y = fmaf(-se, q.y, ty.y);
         }
         if (hasa) out[oa + c] = o.x;
         if (hasb) out[oa + n + c] = o.y;
@@ -362,25 +471,33 @@ ct_rows_inv(const float2* __restrict__ wt, const float* bvec, float* out,
 // ---------------------------------------------------------------------------
 
 // Slices per group of passes.  A group's Wt intermediate (about 144 MB by
-// default: 8 slices of 1454^2) stays resident in the memory-side cache
-// between the three passes, which measured 14 % faster at 32 x 1454^2 than
-// one group of all slices.  TOMOCAM_HIP_TOEPLITZ_SLICES overrides (0: all).
-static int toeplitz_group_slices(int nslices, int H, int rp) {
+// default: 8 slices of 1454^2 in FP32) stays resident in the memory-side
+// cache between the three passes, which measured 14 % faster at 32 x 1454^2
+// than one group of all slices.  TOMOCAM_HIP_TOEPLITZ_SLICES overrides (0:
+// all).
+static int toeplitz_group_slices(int nslices, int H, int rp, int elem_bytes) {
   const char* e = std::getenv("TOMOCAM_HIP_TOEPLITZ_SLICES");
   if (e) {
     const int v = std::atoi(e);
     return v > 0 ? std::min(v, nslices) : nslices;
   }
-  const double per = 8.0 * (double)H * rp;
+  const double per = (double)elem_bytes * H * rp;
   const int g = (int)std::max(1.0, std::floor(144e6 / per));
   return std::min(g, nslices);
 }
 
-template <class P>
-static void run_ct(const float* x, const float* b, float* out, float2* wt,
-                   const float* spec, int n, int H, int rp, int nslices,
+// Slices per group for the FP16-storage passes (sizes the per-column scale
+// buffer).
+int toeplitz_half_group(int nslices, int H, int rp) {
+  return toeplitz_group_slices(nslices, H, rp, 4);
+}
+
+template <class P, bool H16>
+static void run_ct(const void* x, const float* b, float* out, void* wt,
+                   const void* spec, int n, int H, int rp, int nslices,
                    int mode, double* partials, hipStream_t stream,
-                   double img_b, double spec_b, const Psf::StepCommit* sc) {
+                   double img_b, double spec_b, const Psf::StepCommit* sc,
+                   const ToeplitzHalf* half) {
   const float2* tw = fft_length(P::L).plan().tw;
   const size_t smem = sizeof(float2) * P::LPW * P::L;
   const size_t smemc = sizeof(float2) * P::LPWC * P::L;
@@ -390,9 +507,12 @@ static void run_ct(const float* x, const float* b, float* out, float2* wt,
   const long img = (long)n * n;
   const int cm = sc ? (sc->xp ? 2 : 1) : 0;
   const int em = cm ? cm + 1 : mode;  // epilogue mode of pass C
-  const int group = toeplitz_group_slices(nslices, H, rp);
-  // per-slice algorithmic bytes
+  const int group = toeplitz_group_slices(nslices, H, rp, H16 ? 4 : 8);
+  // per-slice algorithmic bytes (FP16 storage: the input and the
+  // intermediates at half width)
   const double ib = img_b / nslices, sb = spec_b / nslices;
+  const double ibx = H16 ? 0.5 * ib : ib, sbw = H16 ? 0.5 * sb : sb;
+  const size_t xel = H16 ? 2 : 4;
   for (int s0 = 0; s0 < nslices; s0 += group) {
     const int ns = std::min(group, nslices - s0);
     const long o = (long)s0 * img;
@@ -402,7 +522,12 @@ static void run_ct(const float* x, const float* b, float* out, float2* wt,
                       sc->dst + o, sc->Tsol + o,
                       sc->Tx ? sc->Tx + o : nullptr, sc->Tdst + o, sc->beta,
                       sc->s};
-    const float* xg = x + o;
+    ToeplitzHalf ha{};
+    if (half) {
+      ha = *half;
+      ha.gsc = half->gsc + s0;
+    }
+    const void* xg = static_cast<const char*>(x) + xel * o;
     const float* bg = b ? b + o : nullptr;
     float* og = (sc ? sc->Tdst : out) + o;
     {
@@ -410,43 +535,43 @@ static void run_ct(const float* x, const float* b, float* out, float2* wt,
       auto launch = [&](auto k) {
         allow_lds((const void*)k, smem);
         hipLaunchKernelGGL(k, dim3(nbr * ns), dim3(P::NT), smem, stream, xg,
-                           wt, tw, n, H, rp, nbr, cp);
+                           wt, tw, n, H, rp, nbr, cp, ha);
       };
-      if (cm == 0) launch(ct_rows_fwd<P, 0>);
-      else if (cm == 1) launch(ct_rows_fwd<P, 1>);
-      else launch(ct_rows_fwd<P, 2>);
+      if (cm == 0) launch(ct_rows_fwd<P, 0, H16>);
+      else if (cm == 1) launch(ct_rows_fwd<P, 1, H16>);
+      else launch(ct_rows_fwd<P, 2, H16>);
       TC_HIP(hipGetLastError());
       // g in, Wt out; committing: sol (and xp) in, dst out
       ktimer_end(KC_TOEPLITZ_ROWS_FWD, stream,
-                 ns * (ib + sb + (cm ? (cm == 2 ? 3.0 : 2.0) * ib : 0.0)));
+                 ns * (ibx + sbw + (cm ? (cm == 2 ? 3.0 : 2.0) * ib : 0.0)));
     }
     {
-      auto k = ct_cols<P>;
+      auto k = ct_cols<P, H16>;
       allow_lds((const void*)k, smemc);
       ktimer_begin(KC_TOEPLITZ_COLS, stream);
       hipLaunchKernelGGL(k, dim3(nbc * ns), dim3(P::NTC), smemc, stream, wt,
                          spec, tw, n, H, rp, nbc,
-                         partials ? partials + (long)s0 * nbc : nullptr);
+                         partials ? partials + (long)s0 * nbc : nullptr, ha);
       TC_HIP(hipGetLastError());
       // Wt in and out; the spectrum is read once per group
       ktimer_end(KC_TOEPLITZ_COLS, stream,
-                 ns * 2.0 * sb + 4.0 * (double)H * P::L);
+                 ns * 2.0 * sbw + (H16 ? 2.0 : 4.0) * (double)H * P::L);
     }
     {
       ktimer_begin(KC_TOEPLITZ_ROWS_INV, stream);
       auto launch = [&](auto k) {
         allow_lds((const void*)k, smem);
         hipLaunchKernelGGL(k, dim3(nbr * ns), dim3(P::NT), smem, stream, wt,
-                           bg, og, tw, n, H, rp, nbr, cp);
+                           bg, og, tw, n, H, rp, nbr, cp, ha);
       };
-      if (em == 0) launch(ct_rows_inv<P, 0>);
-      else if (em == 1) launch(ct_rows_inv<P, 1>);
-      else if (em == 2) launch(ct_rows_inv<P, 2>);
-      else launch(ct_rows_inv<P, 3>);
+      if (em == 0) launch(ct_rows_inv<P, 0, H16>);
+      else if (em == 1) launch(ct_rows_inv<P, 1, H16>);
+      else if (em == 2) launch(ct_rows_inv<P, 2, H16>);
+      else launch(ct_rows_inv<P, 3, H16>);
       TC_HIP(hipGetLastError());
       // Wt in, out written; plus b (mode 1) or Tsol (and Tx) when committing
       const double extra = em == 1 || em == 2 ? 1.0 : (em == 3 ? 2.0 : 0.0);
-      ktimer_end(KC_TOEPLITZ_ROWS_INV, stream, ns * (sb + ib * (1.0 + extra)));
+      ktimer_end(KC_TOEPLITZ_ROWS_INV, stream, ns * (sbw + ib * (1.0 + extra)));
     }
   }
 }
@@ -468,39 +593,45 @@ int toeplitz_ct_lpw(int M) {
   }
 }
 
+template <bool H16>
+static bool apply_ct(int M, const void* x, const float* b, float* out,
+                     void* wt, const void* spec, int n, int H, int rp,
+                     int nslices, int mode, double* partials,
+                     hipStream_t stream, double img_b, double spec_b,
+                     const Psf::StepCommit* sc, const ToeplitzHalf* half) {
+#define TC_RUN_CT(PL)                                                        \
+  run_ct<PL, H16>(x, b, out, wt, spec, n, H, rp, nslices, mode, partials,   \
+                  stream, img_b, spec_b, sc, half);                         \
+  return true
+  switch (M) {
+    case 750: TC_RUN_CT(Plan750);
+    case 1458: TC_RUN_CT(Plan1458);
+    case 2250: TC_RUN_CT(Plan2250);
+    case 2916: TC_RUN_CT(Plan2916);
+    case 4500: TC_RUN_CT(Plan4500);
+    case 5832: TC_RUN_CT(Plan5832);
+    default: return false;
+  }
+#undef TC_RUN_CT
+}
+
 bool toeplitz_apply_ct(int M, const float* x, const float* b, float* out,
                        float2* wt, const float* spec, int n, int H, int rp,
                        int nslices, int mode, double* partials,
                        hipStream_t stream, double img_b, double spec_b,
                        const Psf::StepCommit* sc) {
-  switch (M) {
-    case 750:
-      run_ct<Plan750>(x, b, out, wt, spec, n, H, rp, nslices, mode, partials,
-                      stream, img_b, spec_b, sc);
-      return true;
-    case 1458:
-      run_ct<Plan1458>(x, b, out, wt, spec, n, H, rp, nslices, mode, partials,
-                       stream, img_b, spec_b, sc);
-      return true;
-    case 2250:
-      run_ct<Plan2250>(x, b, out, wt, spec, n, H, rp, nslices, mode, partials,
-                       stream, img_b, spec_b, sc);
-      return true;
-    case 2916:
-      run_ct<Plan2916>(x, b, out, wt, spec, n, H, rp, nslices, mode, partials,
-                       stream, img_b, spec_b, sc);
-      return true;
-    case 4500:
-      run_ct<Plan4500>(x, b, out, wt, spec, n, H, rp, nslices, mode, partials,
-                       stream, img_b, spec_b, sc);
-      return true;
-    case 5832:
-      run_ct<Plan5832>(x, b, out, wt, spec, n, H, rp, nslices, mode, partials,
-                       stream, img_b, spec_b, sc);
-      return true;
-    default:
-      return false;
-  }
+  return apply_ct<false>(M, x, b, out, wt, spec, n, H, rp, nslices, mode,
+                         partials, stream, img_b, spec_b, sc, nullptr);
+}
+
+bool toeplitz_apply_ct_half(int M, const void* x16, float* out, void* wt,
+                            const void* spec16, int n, int H, int rp,
+                            int nslices, double* partials, hipStream_t stream,
+                            double img_b, double spec_b,
+                            const Psf::StepCommit* sc,
+                            const ToeplitzHalf& half) {
+  return apply_ct<true>(M, x16, nullptr, out, wt, spec16, n, H, rp, nslices,
+                        0, partials, stream, img_b, spec_b, sc, &half);
 }
 
 }  // namespace tc

@@ -96,6 +96,18 @@ class PointSpreadFunction:
                   C.byref(dot) if with_dot else None, _stream())
         return (out, dot.value) if with_dot else out
 
+    def convolve_mixed(self, x: torch.Tensor, with_dot: bool = False):
+        """T x with x rounded to FP16 under a per-slice power-of-two scale and
+        FP16 intermediates (the operator of the mixed-precision MBIR form);
+        optionally also x16 . T x16."""
+        x = self._check(x, "x")
+        out = torch.empty_like(x)
+        dot = C.c_double()
+        _lib.call("tc_toeplitz_apply_mixed", self._h, _dev(x, "x"),
+                  _dev(out, "out"), x.shape[0],
+                  C.byref(dot) if with_dot else None, _stream())
+        return (out, dot.value) if with_dot else out
+
     def gradient2(self, x: torch.Tensor, sinoT: torch.Tensor) -> torch.Tensor:
         """T x - sinoT (tomocam::gradient2, src/gradient2.cpp:62)."""
         x = self._check(x, "x")
@@ -328,12 +340,24 @@ def pad2d(img: torch.Tensor, padding: int) -> torch.Tensor:
 def mbir(sino: torch.Tensor, angles, center: float, num_iters: int = 100,
          sigma: float = 100.0, tol: float = 1e-4, xtol: float = 1e-4,
          refresh_every: int = 10, verbose: bool = False,
-         guess: Optional[torch.Tensor] = None):
+         guess: Optional[torch.Tensor] = None, precision: str = "fp32",
+         lipschitz_iters: int = 0):
     """tomocam::mbir2 on a device-resident sinogram [S, P, N] -> [S, N, N].
 
     ``guess`` ([S, N, N]) replaces the FBP initial guess.  Returns (recon,
     info) with info = lipschitz, step, sino_norm, max_sino, padded width and
-    accepted iterations."""
+    accepted iterations.
+
+    ``precision="mixed"`` runs the mixed-precision form (FP16 gradient and
+    Toeplitz intermediates, FP32 state and FFT arithmetic, FP64 dots, T sol
+    refreshed in FP32 every ``refresh_every`` iterations); info then also
+    has ``saturated`` (slice-attempts whose FP16 gradient clipped) and
+    ``backtracks``.  ``lipschitz_iters > 0`` (mixed only) takes L from that
+    many power-iteration steps on T instead of max(T 1)."""
+    if precision not in ("fp32", "mixed"):
+        raise ValueError("precision must be 'fp32' or 'mixed'")
+    if lipschitz_iters and precision != "mixed":
+        raise ValueError("lipschitz_iters is a mixed-precision option")
     sino = _as3d(sino)
     S, P, N = sino.shape
     ang = _angles(angles)
@@ -346,6 +370,16 @@ def mbir(sino: torch.Tensor, angles, center: float, num_iters: int = 100,
             raise ValueError(f"guess must be [{S}, {N}, {N}]")
         x0 = _dev(guess, "guess")
     out = torch.empty((S, N, N), dtype=torch.float32, device=sino.device)
+    if precision == "mixed":
+        info = np.zeros(8, dtype=np.float64)
+        _lib.call("tc_mbir2_mixed", _dev(sino, "sino"), x0, _dev(out, "recon"),
+                  S, P, N, ang.ctypes.data_as(C.c_void_p), float(center),
+                  int(num_iters), float(sigma), int(refresh_every),
+                  int(lipschitz_iters), int(bool(verbose)),
+                  info.ctypes.data_as(_lib.c_double_p), _stream())
+        keys = ("lipschitz", "step", "sino_norm", "max_sino", "padded",
+                "iterations", "saturated", "backtracks")
+        return out, dict(zip(keys, info.tolist()))
     info = np.zeros(6, dtype=np.float64)
     _lib.call("tc_mbir2", _dev(sino, "sino"), x0, _dev(out, "recon"), S, P, N,
               ang.ctypes.data_as(C.c_void_p), float(center), int(num_iters),
