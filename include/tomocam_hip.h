@@ -125,6 +125,17 @@ int tc_psf_info(tc_psf p, int* n, int* fft_size);
 int tc_toeplitz_apply(tc_psf p, const float* d_x, float* d_out, int nslices,
                       double* h_xTx, void* stream);
 
+/* Autotuner of the Toeplitz passes (no counterpart in the reference): times
+ * one application for each slice-group size 1, 2, 4, ... and nslices (the
+ * group's intermediate stays in the memory-side cache between the three
+ * passes) on scratch data, FP32 or the mixed-precision passes, and keeps
+ * the fastest for this width, FFT length and slice count for the rest of
+ * the process.  *group (optional) receives the choice, -1 if the shape was
+ * tuned before, 0 if there is nothing to tune (no compile-time plan, or one
+ * slice).  tc_mbir2 / tc_mbir2_mixed tune on first use unless the
+ * environment sets TOMOCAM_HIP_AUTOTUNE=0.  Synchronises `stream`. */
+int tc_psf_tune(tc_psf p, int nslices, int mixed, int* group, void* stream);
+
 /* Mixed-precision T x (no counterpart in the reference): x rounded to FP16
  * under a per-slice power-of-two scale, the three passes computing in FP32
  * with FP16 intermediates and FP16 spectrum (the operator the mixed MBIR
@@ -247,10 +258,13 @@ int tc_mbir(const float* d_sino, const float* d_x0, float* d_recon,
  * sizes, angle sets and slice counts in one call, each exactly tc_mbir2
  * with an FBP initial guess.  Stack k: d_sinos[k] [nslices[k]][nproj[k]]
  * [nrays[k]] -> d_recons[k] [nslices[k]][nrays[k]][nrays[k]], angles
- * h_angles[k] (nproj[k] floats, host), centre centers[k].  Stacks run in
- * order on `stream`; NUFFT plans are cached by geometry.  h_info (optional):
- * 6 doubles per stack as for tc_mbir2.  On error the stack index is in the
- * message and the later stacks are not run. */
+ * h_angles[k] (nproj[k] floats, host), centre centers[k].  Stacks run on
+ * TOMOCAM_HIP_BATCH_STREAMS (default 4) streams of their own at once, one
+ * host thread each, largest first (1: in order on `stream`); the call waits
+ * for `stream` first and returns when every stack is done.  NUFFT plans are
+ * cached by geometry.  h_info (optional): 6 doubles per stack as for
+ * tc_mbir2.  On error the message names the lowest failing stack index and
+ * stacks not yet started are not run. */
 int tc_mbir2_batch(int nstacks, const float* const* d_sinos,
                    float* const* d_recons, const int* nslices,
                    const int* nproj, const int* nrays,

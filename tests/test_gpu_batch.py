@@ -60,3 +60,42 @@ def test_empty_batch(torch_cuda):
     from tomocam_amd import gpu
     recs, infos = gpu.mbir_batch([], 5, 10.0)
     assert recs == [] and infos == []
+
+
+def test_batch_concurrent_streams_equal_sequential(torch_cuda, monkeypatch):
+    """Stacks on 4 concurrent streams (the default) against the same batch
+    run in order on one stream: identical reconstructions.  Sizes with
+    compile-time FFT plans (padded 362 and 726) and one without (padded
+    44), largest first in the concurrent schedule."""
+    from tomocam_amd import gpu
+    stacks = []
+    for k, (S, P, N) in enumerate([(1, 60, 256), (3, 90, 512), (2, 24, 32),
+                                   (2, 45, 256), (1, 120, 512)]):
+        ang = PH.angles(P)
+        img = torch.tensor(np.stack([PH.random_ellipses(N, 700 + 10 * k + s)
+                                     for s in range(S)]),
+                           dtype=torch.float32).cuda()
+        stacks.append((gpu.project(img, ang), ang, N / 2.0))
+    announce("mbir2 batch sequential (1 stream)")
+    monkeypatch.setenv("TOMOCAM_HIP_BATCH_STREAMS", "1")
+    seq, infos_seq = gpu.mbir_batch(stacks, 8, 100.0)
+    announce("mbir2 batch concurrent (4 streams)")
+    monkeypatch.setenv("TOMOCAM_HIP_BATCH_STREAMS", "4")
+    con, infos_con = gpu.mbir_batch(stacks, 8, 100.0)
+    for k in range(len(stacks)):
+        assert infos_con[k]["padded"] == infos_seq[k]["padded"]
+        assert infos_con[k]["iterations"] == 8
+        # same kernels and group sizes: equal up to FP64 dot order
+        assert rel_l2(con[k].cpu().numpy(), seq[k].cpu().numpy()) < 1e-6, k
+
+
+def test_batch_concurrent_error_names_lowest_stack(torch_cuda, monkeypatch):
+    from tomocam_amd import gpu, _lib
+    st = _stacks(torch_cuda)
+    stacks = [(s, a, c) for s, a, c, _ in st]
+    stacks[1] = (torch.zeros_like(stacks[1][0]), stacks[1][1], stacks[1][2])
+    stacks[3] = (torch.zeros_like(stacks[3][0]), stacks[3][1], stacks[3][2])
+    monkeypatch.setenv("TOMOCAM_HIP_BATCH_STREAMS", "4")
+    announce("mbir2 concurrent batch error path")
+    with pytest.raises((ValueError, _lib.TomocamError), match="stack [13]"):
+        gpu.mbir_batch(stacks, 2, 10.0)

@@ -89,3 +89,30 @@ def test_psf_roundtrip_and_info(torch_cuda):
     assert psf.fft_size >= 39 and psf.fft_size % 2 == 0
     back = psf.array().cpu().numpy()
     assert np.array_equal(back, ps.astype(np.float32))
+
+
+def test_toeplitz_group_autotuner(torch_cuda):
+    """The slice-group autotuner picks one of its candidates, remembers it,
+    and does not change T x (slices are independent; x . T x only changes
+    in its FP64 summation order)."""
+    torch = torch_cuda
+    from tomocam_amd import gpu
+    n, S = 362, 5
+    ps = _even_psf(n, 11)
+    psf = gpu.PointSpreadFunction(
+        psf_array=torch.tensor(ps, dtype=torch.float32).cuda())
+    x = torch.tensor(np.random.default_rng(2).standard_normal((S, n, n)),
+                     dtype=torch.float32).cuda()
+    announce("toeplitz autotuner n=362 S=5")
+    before, d0 = psf.convolve(x, with_dot=True)
+    g = psf.tune(S)
+    assert g in (1, 2, 4, 5)
+    assert psf.tune(S) == -1
+    after, d1 = psf.convolve(x, with_dot=True)
+    assert torch.equal(before, after)
+    assert abs(d1 - d0) <= 1e-12 * abs(d0)
+    assert psf.tune(1) == 0
+    assert psf.tune(S, mixed=True) in (1, 2, 4, 5)
+    small = gpu.PointSpreadFunction(
+        psf_array=torch.tensor(_even_psf(181, 3), dtype=torch.float32).cuda())
+    assert small.tune(S) == 0  # FFT length 378: no compile-time passes

@@ -1,6 +1,11 @@
 // extern "C" entry points (declared in include/tomocam_hip.h).
+#include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <memory>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "driver.h"
@@ -198,6 +203,15 @@ int tc_toeplitz_apply(tc_psf p, const float* d_x, float* d_out, int nslices,
     TC_HIP(hipMemcpyAsync(h_xTx, parts.as<double>() + np, sizeof(double),
                           hipMemcpyDeviceToHost, s));
     TC_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int tc_psf_tune(tc_psf p, int nslices, int mixed, int* group, void* stream) {
+  return guarded([&] {
+    tc::Psf& P = get_psf(p);
+    TC_REQUIRE(nslices >= 1, "Toeplitz: need at least one slice");
+    const int g = P.tune_groups(nslices, mixed != 0, as_stream(stream));
+    if (group) *group = g;
   });
 }
 
@@ -427,19 +441,91 @@ int tc_mbir2_batch(int nstacks, const float* const* d_sinos,
     o.num_iters = num_iters;
     o.sigma = sigma;
     o.refresh_every = refresh_every;
-    for (int k = 0; k < nstacks; ++k) {
-      try {
-        TC_REQUIRE(d_sinos[k] && d_recons[k] && h_angles[k], "null argument");
-        tc::Mbir2Result r = tc::mbir2(d_sinos[k], nullptr, d_recons[k],
-                                      nslices[k], nproj[k], nrays[k],
-                                      h_angles[k], centers[k], o,
-                                      as_stream(stream));
-        fill_info(r, h_info ? h_info + 6 * (long)k : nullptr);
-      } catch (const tc::Error& e) {
-        throw tc::Error(e.code, "stack " + std::to_string(k) + ": " +
-                                      e.what());
+    auto run_one = [&](int k, hipStream_t s) {
+      TC_REQUIRE(d_sinos[k] && d_recons[k] && h_angles[k], "null argument");
+      tc::Mbir2Result r = tc::mbir2(d_sinos[k], nullptr, d_recons[k],
+                                    nslices[k], nproj[k], nrays[k],
+                                    h_angles[k], centers[k], o, s);
+      fill_info(r, h_info ? h_info + 6 * (long)k : nullptr);
+    };
+    // Stacks run on `workers` streams at once (TOMOCAM_HIP_BATCH_STREAMS,
+    // default 4), one host thread each, largest stacks first: a small stack
+    // alone leaves most of the GPU idle between its line-search
+    // synchronisations.
+    const char* ev = std::getenv("TOMOCAM_HIP_BATCH_STREAMS");
+    int workers = ev ? std::atoi(ev) : 4;
+    workers = std::max(1, std::min(workers, nstacks));
+    if (workers == 1) {
+      for (int k = 0; k < nstacks; ++k) {
+        try {
+          run_one(k, as_stream(stream));
+        } catch (const tc::Error& e) {
+          throw tc::Error(e.code, "stack " + std::to_string(k) + ": " +
+                                        e.what());
+        }
       }
+      return;
     }
+    TC_HIP(hipStreamSynchronize(as_stream(stream)));  // inputs are ready
+    // timings taken while other stacks run would be noise: concurrent
+    // stacks use tuned sizes from earlier calls, or the default rule
+    o.autotune = false;
+    int dev = 0;
+    TC_HIP(hipGetDevice(&dev));
+    std::vector<int> order(nstacks);
+    for (int k = 0; k < nstacks; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      return (double)nslices[a] * nrays[a] * nrays[a] >
+             (double)nslices[b] * nrays[b] * nrays[b];
+    });
+    std::atomic<int> next{0};
+    std::atomic<bool> stop{false};
+    std::mutex emu;
+    int err_stack = -1, err_code = TC_OK;
+    std::string err_msg;
+    auto fail = [&](int k, int code, const std::string& msg) {
+      std::lock_guard<std::mutex> lk(emu);
+      if (err_code == TC_OK || (k >= 0 && (err_stack < 0 || k < err_stack))) {
+        err_stack = k;
+        err_code = code;
+        err_msg = msg;
+      }
+      stop = true;
+    };
+    auto work = [&](int w) {
+      int k = -1;
+      try {
+        TC_HIP(hipSetDevice(dev));
+        tc::set_plan_slot(w + 1);
+        hipStream_t s = nullptr;
+        TC_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        struct StreamFree {
+          hipStream_t s;
+          ~StreamFree() { (void)hipStreamDestroy(s); }
+        } sf{s};
+        while (!stop) {
+          const int i = next.fetch_add(1);
+          if (i >= nstacks) break;
+          k = order[i];
+          run_one(k, s);
+        }
+      } catch (const tc::Error& e) {
+        fail(k, e.code, e.what());
+      } catch (const std::bad_alloc&) {
+        fail(k, TC_ERR_NOMEM, "host allocation failed");
+      } catch (const std::exception& e) {
+        fail(k, TC_ERR_INTERNAL, e.what());
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int w = 0; w < workers; ++w) pool.emplace_back(work, w);
+    for (auto& t : pool) t.join();
+    if (err_code != TC_OK)
+      throw tc::Error(err_code, (err_stack >= 0 ? "stack " +
+                                                       std::to_string(err_stack) +
+                                                       ": "
+                                                 : std::string()) +
+                                    err_msg);
   });
 }
 

@@ -34,6 +34,9 @@ struct Mbir2Options {
   // > 0: L from that many power-iteration steps on T (FP64 dots), starting
   // from the all-ones image, instead of the reference's max(T 1)
   int lipschitz_iters = 0;
+  // time the Toeplitz slice-group sizes once per shape (Psf::tune_groups);
+  // TOMOCAM_HIP_AUTOTUNE=0 turns it off process-wide
+  bool autotune = true;
 };
 
 struct Mbir2Result {

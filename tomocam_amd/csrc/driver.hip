@@ -1,5 +1,6 @@
 // Pre/post-processing kernels and the device-resident mbir2 pipeline.
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <utility>
 
@@ -151,6 +152,13 @@ Mbir2Result mbir2(const float* sino, const float* x0, float* recon, int S,
   polar_psf(h_angles, P, n2, bpsf.as<float>(), stream);
   Psf psf(bpsf.as<float>(), n2, stream);
   bpsf.release();
+  // slice-group size of the Toeplitz passes, timed once per shape and
+  // precision (TOMOCAM_HIP_AUTOTUNE=0 keeps the 144 MB rule)
+  {
+    const char* at = std::getenv("TOMOCAM_HIP_AUTOTUNE");
+    if (o.autotune && !(at && at[0] == '0'))
+      psf.tune_groups(S, o.mixed, stream);
+  }
 
   // L = max(Re(A^H A 1)/n^3) = max(T 1); the qGGMRF Hessian term never
   // reaches the host in the reference (src/mbir2.cpp:101-112,

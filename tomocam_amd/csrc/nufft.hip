@@ -855,19 +855,32 @@ void RayDft::apply(const void* in, bool in_real, void* out, bool out_real,
 // plan caches
 // ---------------------------------------------------------------------------
 
+// A PolarNufft keeps scratch (the fine grid) between calls, so threads that
+// run concurrently use plans of their own: the cache key holds the calling
+// thread's plan slot (0 unless set_plan_slot gave it another).
+static thread_local int t_plan_slot = 0;
+void set_plan_slot(int slot) { t_plan_slot = slot; }
+
 PolarNufft& nufft_plan(const float* h_angles, int nproj, int npix,
                        int nmodes) {
   static std::mutex mu;
-  using Key = std::tuple<int, int, int, int, std::vector<float>>;
+  using Key = std::tuple<int, int, int, int, int, std::vector<float>>;
   static std::map<Key, std::unique_ptr<PolarNufft>> cache;
   int dev = 0;
   TC_HIP(hipGetDevice(&dev));
   std::vector<float> ang(h_angles, h_angles + nproj);
-  Key key(dev, nproj, npix, nmodes, ang);
+  const int slot = t_plan_slot;
+  Key key(slot, dev, nproj, npix, nmodes, ang);
   std::lock_guard<std::mutex> lk(mu);
   auto it = cache.find(key);
   if (it == cache.end()) {
-    if (cache.size() >= 16) cache.clear();
+    // at most 16 plans per slot: drop this slot's (only this thread uses
+    // them, and it holds none while it asks for a plan)
+    int mine = 0;
+    for (auto& kv : cache) mine += std::get<0>(kv.first) == slot;
+    if (mine >= 16)
+      for (auto i = cache.begin(); i != cache.end();)
+        i = std::get<0>(i->first) == slot ? cache.erase(i) : std::next(i);
     it = cache
              .emplace(key, std::make_unique<PolarNufft>(h_angles, nproj, npix,
                                                         nmodes))

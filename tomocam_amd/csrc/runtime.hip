@@ -18,8 +18,10 @@ struct KTimer {
   bool enabled[KC_COUNT] = {};
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs[KC_COUNT];
   double bytes[KC_COUNT] = {};
-  hipEvent_t open[KC_COUNT] = {};
 };
+
+// the event opened by ktimer_begin, per thread (concurrent callers)
+static thread_local hipEvent_t t_open[KC_COUNT] = {};
 
 static KTimer& kt() {
   static KTimer k;
@@ -38,19 +40,19 @@ void ktimer_begin(int kc, hipStream_t stream) {
   hipEvent_t e;
   TC_HIP(hipEventCreate(&e));
   TC_HIP(hipEventRecord(e, stream));
-  k.open[kc] = e;
+  t_open[kc] = e;
 }
 
 void ktimer_end(int kc, hipStream_t stream, double bytes) {
   KTimer& k = kt();
-  if (!k.enabled[kc] || !k.open[kc]) return;
+  if (!k.enabled[kc] || !t_open[kc]) return;
   hipEvent_t e;
   TC_HIP(hipEventCreate(&e));
   TC_HIP(hipEventRecord(e, stream));
   std::lock_guard<std::mutex> lk(k.mu);
-  k.pairs[kc].emplace_back(k.open[kc], e);
+  k.pairs[kc].emplace_back(t_open[kc], e);
   k.bytes[kc] += bytes;
-  k.open[kc] = nullptr;
+  t_open[kc] = nullptr;
 }
 
 // Total milliseconds, launch count and algorithmic bytes of class kc since

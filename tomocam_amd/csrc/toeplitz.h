@@ -73,6 +73,14 @@ class Psf {
   // the mixed-precision forms).
   bool has_fused() const;
 
+  // Autotuner: times one application of the compile-time passes for each
+  // candidate slice-group size (1, 2, 4, ... and all `nslices`) on scratch
+  // data and keeps the fastest for this (n, M, precision, nslices) for the
+  // rest of the process.  Returns the chosen group size, -1 if this shape
+  // was tuned before, 0 if there is nothing to tune (no compile-time plan,
+  // or a single slice).
+  int tune_groups(int nslices, bool half, hipStream_t stream);
+
   // Grows the workspace for `nslices` slices.
   void reserve(int nslices);
 

@@ -3,6 +3,7 @@
 #include <hip/hip_fp16.h>
 
 #include <cmath>
+#include <vector>
 
 #include "fft_device.h"
 #include "fft_ops.h"
@@ -342,6 +343,72 @@ bool Psf::run_half(const void* g16, const float* gsc, float* out,
 }
 
 bool Psf::has_fused() const { return toeplitz_ct_lpw(M_) > 0; }
+
+void toeplitz_set_group(int H, int rp, int elem_bytes, int nslices,
+                        int group);
+bool toeplitz_group_tuned(int H, int rp, int elem_bytes, int nslices);
+
+__global__ void fill_float_kernel(float* p, long n, float v) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+int Psf::tune_groups(int nslices, bool half, hipStream_t stream) {
+  if (!has_fused() || nslices < 2) return 0;
+  const int eb = half ? 4 : 8;
+  if (toeplitz_group_tuned(H_, rp_, eb, nslices)) return -1;
+  const long plane = (long)n_ * n_;
+  const long tot = plane * nslices;
+  // scratch operands: the passes' time does not depend on the values
+  DevBuf x(sizeof(float) * (size_t)tot), out(sizeof(float) * (size_t)tot);
+  DevBuf gsc;
+  TC_HIP(hipMemsetAsync(x.as<void>(), 0, sizeof(float) * (size_t)tot, stream));
+  if (half) {
+    gsc.alloc(sizeof(float) * (size_t)nslices);
+    hipLaunchKernelGGL(fill_float_kernel, dim3(cdiv(nslices, 256)), dim3(256),
+                       0, stream, gsc.as<float>(), (long)nslices, 1.0f);
+    TC_HIP(hipGetLastError());
+  }
+  std::vector<int> cands;
+  for (int g = 1; g < nslices; g *= 2) cands.push_back(g);
+  cands.push_back(nslices);
+  hipEvent_t e0, e1;
+  TC_HIP(hipEventCreate(&e0));
+  TC_HIP(hipEventCreate(&e1));
+  int best = 0;
+  float best_ms = 0.f;
+  try {
+    for (int g : cands) {
+      toeplitz_set_group(H_, rp_, eb, nslices, g);
+      float ms = 0.f;
+      for (int rep = 0; rep < 2; ++rep) {  // first run warms caches
+        TC_HIP(hipEventRecord(e0, stream));
+        if (half)
+          apply_half(x.as<void>(), gsc.as<float>(), out.as<float>(), nslices,
+                     nullptr, stream);
+        else
+          apply(x.as<float>(), nullptr, out.as<float>(), nslices, 0, nullptr,
+                stream);
+        TC_HIP(hipEventRecord(e1, stream));
+        TC_HIP(hipEventSynchronize(e1));
+        TC_HIP(hipEventElapsedTime(&ms, e0, e1));
+      }
+      if (best == 0 || ms < best_ms) {
+        best = g;
+        best_ms = ms;
+      }
+    }
+  } catch (...) {
+    toeplitz_set_group(H_, rp_, eb, nslices, 0);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  toeplitz_set_group(H_, rp_, eb, nslices, best);
+  return best;
+}
 
 bool Psf::apply_half(const void* g16, const float* gsc, float* out,
                      int nslices, double* dot_partials, hipStream_t stream) {

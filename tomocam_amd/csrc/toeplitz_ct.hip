@@ -2,6 +2,9 @@
 // sizes; other sizes use the generic kernels in toeplitz.hip.
 #include <algorithm>
 #include <cstdlib>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include <hip/hip_fp16.h>
 
@@ -470,16 +473,50 @@ ct_rows_inv(const void* __restrict__ wtv, const float* bvec, float* out,
 
 // ---------------------------------------------------------------------------
 
-// Slices per group of passes.  A group's Wt intermediate (about 144 MB by
-// default: 8 slices of 1454^2 in FP32) stays resident in the memory-side
-// cache between the three passes, which measured 14 % faster at 32 x 1454^2
-// than one group of all slices.  TOMOCAM_HIP_TOEPLITZ_SLICES overrides (0:
-// all).
+// Group sizes picked by the autotuner (Psf::tune_groups), keyed by
+// (device, H, rp, element bytes, slices).
+static std::mutex g_group_mu;
+static std::map<std::tuple<int, int, int, int, int>, int> g_group_tuned;
+
+static std::tuple<int, int, int, int, int> group_key(int H, int rp,
+                                                     int elem_bytes,
+                                                     int nslices) {
+  int dev = 0;
+  TC_HIP(hipGetDevice(&dev));
+  return std::make_tuple(dev, H, rp, elem_bytes, nslices);
+}
+
+// group <= 0 forgets the entry
+void toeplitz_set_group(int H, int rp, int elem_bytes, int nslices,
+                        int group) {
+  std::lock_guard<std::mutex> lk(g_group_mu);
+  const auto key = group_key(H, rp, elem_bytes, nslices);
+  if (group > 0)
+    g_group_tuned[key] = group;
+  else
+    g_group_tuned.erase(key);
+}
+
+bool toeplitz_group_tuned(int H, int rp, int elem_bytes, int nslices) {
+  std::lock_guard<std::mutex> lk(g_group_mu);
+  return g_group_tuned.count(group_key(H, rp, elem_bytes, nslices)) != 0;
+}
+
+// Slices per group of passes.  A group's Wt intermediate stays resident in
+// the memory-side cache between the three passes; about 144 MB per group
+// (8 slices of 1454^2 in FP32) measured 14 % faster at 32 x 1454^2 than one
+// group of all slices.  Order of precedence: TOMOCAM_HIP_TOEPLITZ_SLICES
+// (0: all), the autotuner's choice for this shape, the 144 MB rule.
 static int toeplitz_group_slices(int nslices, int H, int rp, int elem_bytes) {
   const char* e = std::getenv("TOMOCAM_HIP_TOEPLITZ_SLICES");
   if (e) {
     const int v = std::atoi(e);
     return v > 0 ? std::min(v, nslices) : nslices;
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_group_mu);
+    auto it = g_group_tuned.find(group_key(H, rp, elem_bytes, nslices));
+    if (it != g_group_tuned.end()) return std::min(it->second, nslices);
   }
   const double per = (double)elem_bytes * H * rp;
   const int g = (int)std::max(1.0, std::floor(144e6 / per));

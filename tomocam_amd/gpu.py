@@ -96,6 +96,15 @@ class PointSpreadFunction:
                   C.byref(dot) if with_dot else None, _stream())
         return (out, dot.value) if with_dot else out
 
+    def tune(self, nslices: int, mixed: bool = False) -> int:
+        """Times the Toeplitz passes' slice-group sizes for ``nslices`` slices
+        and keeps the fastest for the process; returns it (-1: tuned before,
+        0: nothing to tune)."""
+        g = C.c_int()
+        _lib.call("tc_psf_tune", self._h, int(nslices), int(bool(mixed)),
+                  C.byref(g), _stream())
+        return g.value
+
     def convolve_mixed(self, x: torch.Tensor, with_dot: bool = False):
         """T x with x rounded to FP16 under a per-slice power-of-two scale and
         FP16 intermediates (the operator of the mixed-precision MBIR form);
