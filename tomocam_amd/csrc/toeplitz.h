@@ -16,11 +16,23 @@
 
 namespace tc {
 
+// Second stream for the Toeplitz passes: slice groups alternate between the
+// caller's stream and `aux` (each with its own workspace half), so one
+// group's row passes (memory-latency bound) overlap the next group's column
+// pass (VALU bound).
+struct ToeplitzStreams {
+  hipStream_t aux;
+  hipEvent_t start, done;
+};
+
 class Psf {
  public:
   // From a real PSF array [2n-1][2n-1] (PSF(m) at index m + n - 1) in device
   // memory.
   Psf(const float* d_psf, int n, hipStream_t stream);
+  ~Psf();
+  Psf(const Psf&) = delete;
+  Psf& operator=(const Psf&) = delete;
 
   int n() const { return n_; }
   int M() const { return M_; }
@@ -74,9 +86,9 @@ class Psf {
   bool has_fused() const;
 
   // Autotuner: times one application of the compile-time passes for each
-  // candidate slice-group size (1, 2, 4, ... and all `nslices`) on scratch
-  // data and keeps the fastest for this (n, M, precision, nslices) for the
-  // rest of the process.  Returns the chosen group size, -1 if this shape
+  // candidate slice-group size (half, once and twice the 144 MB rule's), on
+  // one stream and alternating over two, on scratch data and keeps the
+  // fastest for this (n, M, precision, nslices) for the rest of the process.  Returns the chosen group size, -1 if this shape
   // was tuned before, 0 if there is nothing to tune (no compile-time plan,
   // or a single slice).
   int tune_groups(int nslices, bool half, hipStream_t stream);
@@ -85,6 +97,10 @@ class Psf {
   void reserve(int nslices);
 
  private:
+  // the two-stream set-up when TOMOCAM_HIP_TOEPLITZ_STREAMS (or the tuner)
+  // asks for two streams for this slice count, else nullptr
+  const ToeplitzStreams* streams_for(int nslices, int elem_bytes);
+  ToeplitzStreams ts_{nullptr, nullptr, nullptr};
   bool run_half(const void* g16, const float* gsc, float* out, int nslices,
                 double* dot_partials, const StepCommit* c, hipStream_t stream);
   int n_, M_, H_, rp_;

@@ -262,14 +262,36 @@ bool toeplitz_apply_ct(int M, const float* x, const float* b, float* out,
                        float2* wt, const float* spec, int n, int H, int rp,
                        int nslices, int mode, double* partials,
                        hipStream_t stream, double img_b, double spec_b,
-                       const Psf::StepCommit* cm);
+                       const Psf::StepCommit* cm, const ToeplitzStreams* ts);
+int toeplitz_streams(int H, int rp, int elem_bytes, int nslices);
+int toeplitz_rule_group(int nslices, int H, int rp, int elem_bytes);
+void toeplitz_set_streams(int H, int rp, int elem_bytes, int nslices,
+                          int streams);
+
+Psf::~Psf() {
+  if (ts_.aux) (void)hipStreamDestroy(ts_.aux);
+  if (ts_.start) (void)hipEventDestroy(ts_.start);
+  if (ts_.done) (void)hipEventDestroy(ts_.done);
+}
+
+const ToeplitzStreams* Psf::streams_for(int nslices, int elem_bytes) {
+  if (toeplitz_streams(H_, rp_, elem_bytes, nslices) < 2) return nullptr;
+  if (!ts_.aux) {
+    // default priority, non-blocking: ordered only by the events below
+    TC_HIP(hipStreamCreateWithFlags(&ts_.aux, hipStreamNonBlocking));
+    TC_HIP(hipEventCreateWithFlags(&ts_.start, hipEventDisableTiming));
+    TC_HIP(hipEventCreateWithFlags(&ts_.done, hipEventDisableTiming));
+  }
+  return &ts_;
+}
 
 bool toeplitz_apply_ct_half(int M, const void* x16, float* out, void* wt,
                             const void* spec16, int n, int H, int rp,
                             int nslices, double* partials, hipStream_t stream,
                             double img_b, double spec_b,
                             const Psf::StepCommit* sc,
-                            const ToeplitzHalf& half);
+                            const ToeplitzHalf& half,
+                            const ToeplitzStreams* ts);
 int toeplitz_half_group(int nslices, int H, int rp);
 
 // max |v[i]| over n floats into out[0] (one workgroup)
@@ -322,7 +344,8 @@ bool Psf::run_half(const void* g16, const float* gsc, float* out,
     TC_HIP(hipGetLastError());
   }
   reserve(nslices);
-  const int group = toeplitz_half_group(nslices, H_, rp_);
+  // per-column scales for two groups in flight (two-stream passes)
+  const int group = 2 * toeplitz_half_group(nslices, H_, rp_);
   if ((size_t)group > cinv_slices_) {
     cinv_.alloc(sizeof(float) * (size_t)group * H_);
     cinv_slices_ = group;
@@ -339,7 +362,8 @@ bool Psf::run_half(const void* g16, const float* gsc, float* out,
   const double spec_b = 8.0 * nslices * (double)n_ * H_;
   return toeplitz_apply_ct_half(M_, g16, c ? c->Tdst : out, work_.as<void>(),
                                 spec16_.as<void>(), n_, H_, rp_, nslices,
-                                dot_partials, stream, img_b, spec_b, c, h);
+                                dot_partials, stream, img_b, spec_b, c, h,
+                                streams_for(nslices, 4));
 }
 
 bool Psf::has_fused() const { return toeplitz_ct_lpw(M_) > 0; }
@@ -369,17 +393,27 @@ int Psf::tune_groups(int nslices, bool half, hipStream_t stream) {
                        0, stream, gsc.as<float>(), (long)nslices, 1.0f);
     TC_HIP(hipGetLastError());
   }
-  std::vector<int> cands;
-  for (int g = 1; g < nslices; g *= 2) cands.push_back(g);
-  cands.push_back(nslices);
+  // (group, streams) candidates around the 144 MB rule's group r: r/2, r,
+  // 2r (capped at nslices), each on one stream and, when there are at
+  // least two groups, alternating over two
+  const int r = toeplitz_rule_group(nslices, H_, rp_, eb);
+  std::vector<std::pair<int, int>> cands;
+  for (int g : {std::max(1, r / 2), r, std::min(2 * r, nslices)}) {
+    bool seen = false;
+    for (auto& c : cands) seen |= c.first == g;
+    if (seen) continue;
+    cands.emplace_back(g, 1);
+    if (g < nslices) cands.emplace_back(g, 2);
+  }
   hipEvent_t e0, e1;
   TC_HIP(hipEventCreate(&e0));
   TC_HIP(hipEventCreate(&e1));
-  int best = 0;
+  int best = 0, best_st = 1;
   float best_ms = 0.f;
   try {
-    for (int g : cands) {
+    for (auto [g, st] : cands) {
       toeplitz_set_group(H_, rp_, eb, nslices, g);
+      toeplitz_set_streams(H_, rp_, eb, nslices, st);
       float ms = 0.f;
       for (int rep = 0; rep < 2; ++rep) {  // first run warms caches
         TC_HIP(hipEventRecord(e0, stream));
@@ -395,11 +429,13 @@ int Psf::tune_groups(int nslices, bool half, hipStream_t stream) {
       }
       if (best == 0 || ms < best_ms) {
         best = g;
+        best_st = st;
         best_ms = ms;
       }
     }
   } catch (...) {
     toeplitz_set_group(H_, rp_, eb, nslices, 0);
+    toeplitz_set_streams(H_, rp_, eb, nslices, 0);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     throw;
@@ -407,6 +443,7 @@ int Psf::tune_groups(int nslices, bool half, hipStream_t stream) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   toeplitz_set_group(H_, rp_, eb, nslices, best);
+  toeplitz_set_streams(H_, rp_, eb, nslices, best_st);
   return best;
 }
 
@@ -435,7 +472,7 @@ bool Psf::apply_commit(const float* g, const StepCommit& c, int nslices,
   const double spec_b = 8.0 * nslices * (double)n_ * H_;
   return toeplitz_apply_ct(M_, g, nullptr, c.Tdst, work_.as<float2>(),
                            spectrum(), n_, H_, rp_, nslices, 0, dot_partials,
-                           stream, img_b, spec_b, &c);
+                           stream, img_b, spec_b, &c, streams_for(nslices, 8));
 }
 
 void Psf::apply(const float* x, const float* b, float* out, int nslices,
@@ -448,7 +485,8 @@ void Psf::apply(const float* x, const float* b, float* out, int nslices,
   const double img_b = 4.0 * nslices * (double)n_ * n_;
   const double spec_b = 8.0 * nslices * (double)n_ * H_;
   if (toeplitz_apply_ct(M_, x, b, out, wt, spectrum(), n_, H_, rp_, nslices,
-                        mode, dot_partials, stream, img_b, spec_b, nullptr))
+                        mode, dot_partials, stream, img_b, spec_b, nullptr,
+                        streams_for(nslices, 8)))
     return;
   const FFTLength& F = fft_length(M_);
   const int npairs = (n_ + 1) / 2;

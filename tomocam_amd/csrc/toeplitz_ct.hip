@@ -473,6 +473,13 @@ ct_rows_inv(const void* __restrict__ wtv, const float* bvec, float* out,
 
 // ---------------------------------------------------------------------------
 
+// the 144 MB rule below
+int toeplitz_rule_group(int nslices, int H, int rp, int elem_bytes) {
+  const double per = (double)elem_bytes * H * rp;
+  const int g = (int)std::max(1.0, std::floor(144e6 / per));
+  return std::min(g, nslices);
+}
+
 // Group sizes picked by the autotuner (Psf::tune_groups), keyed by
 // (device, H, rp, element bytes, slices).
 static std::mutex g_group_mu;
@@ -502,6 +509,28 @@ bool toeplitz_group_tuned(int H, int rp, int elem_bytes, int nslices) {
   return g_group_tuned.count(group_key(H, rp, elem_bytes, nslices)) != 0;
 }
 
+// Streams for the passes (1 or 2), keyed like the group sizes.
+static std::map<std::tuple<int, int, int, int, int>, int> g_streams_tuned;
+
+void toeplitz_set_streams(int H, int rp, int elem_bytes, int nslices,
+                          int streams) {
+  std::lock_guard<std::mutex> lk(g_group_mu);
+  const auto key = group_key(H, rp, elem_bytes, nslices);
+  if (streams > 0)
+    g_streams_tuned[key] = streams;
+  else
+    g_streams_tuned.erase(key);
+}
+
+// TOMOCAM_HIP_TOEPLITZ_STREAMS, else the tuner's choice, else 1
+int toeplitz_streams(int H, int rp, int elem_bytes, int nslices) {
+  const char* e = std::getenv("TOMOCAM_HIP_TOEPLITZ_STREAMS");
+  if (e) return std::atoi(e) >= 2 ? 2 : 1;
+  std::lock_guard<std::mutex> lk(g_group_mu);
+  auto it = g_streams_tuned.find(group_key(H, rp, elem_bytes, nslices));
+  return it != g_streams_tuned.end() ? it->second : 1;
+}
+
 // Slices per group of passes.  A group's Wt intermediate stays resident in
 // the memory-side cache between the three passes; about 144 MB per group
 // (8 slices of 1454^2 in FP32) measured 14 % faster at 32 x 1454^2 than one
@@ -518,9 +547,7 @@ static int toeplitz_group_slices(int nslices, int H, int rp, int elem_bytes) {
     auto it = g_group_tuned.find(group_key(H, rp, elem_bytes, nslices));
     if (it != g_group_tuned.end()) return std::min(it->second, nslices);
   }
-  const double per = (double)elem_bytes * H * rp;
-  const int g = (int)std::max(1.0, std::floor(144e6 / per));
-  return std::min(g, nslices);
+  return toeplitz_rule_group(nslices, H, rp, elem_bytes);
 }
 
 // Slices per group for the FP16-storage passes (sizes the per-column scale
@@ -532,9 +559,9 @@ int toeplitz_half_group(int nslices, int H, int rp) {
 template <class P, bool H16>
 static void run_ct(const void* x, const float* b, float* out, void* wt,
                    const void* spec, int n, int H, int rp, int nslices,
-                   int mode, double* partials, hipStream_t stream,
+                   int mode, double* partials, hipStream_t stream0,
                    double img_b, double spec_b, const Psf::StepCommit* sc,
-                   const ToeplitzHalf* half) {
+                   const ToeplitzHalf* half, const ToeplitzStreams* ts) {
   const float2* tw = fft_length(P::L).plan().tw;
   const size_t smem = sizeof(float2) * P::LPW * P::L;
   const size_t smemc = sizeof(float2) * P::LPWC * P::L;
@@ -550,9 +577,19 @@ static void run_ct(const void* x, const float* b, float* out, void* wt,
   const double ib = img_b / nslices, sb = spec_b / nslices;
   const double ibx = H16 ? 0.5 * ib : ib, sbw = H16 ? 0.5 * sb : sb;
   const size_t xel = H16 ? 2 : 4;
-  for (int s0 = 0; s0 < nslices; s0 += group) {
+  const bool dual = ts != nullptr && nslices > group;
+  if (dual) {
+    TC_HIP(hipEventRecord(ts->start, stream0));
+    TC_HIP(hipStreamWaitEvent(ts->aux, ts->start, 0));
+  }
+  const long wslab = (long)group * H * rp;  // workspace elements per group
+  for (int s0 = 0, gi = 0; s0 < nslices; s0 += group, ++gi) {
     const int ns = std::min(group, nslices - s0);
     const long o = (long)s0 * img;
+    // odd groups: the second stream and the second workspace half
+    const bool odd = dual && (gi & 1);
+    hipStream_t stream = odd ? ts->aux : stream0;
+    void* wtg = static_cast<char*>(wt) + (odd ? wslab * (H16 ? 4 : 8) : 0);
     CommitPtrs cp{};
     if (sc)
       cp = CommitPtrs{sc->sol + o, sc->xp ? sc->xp + o : nullptr,
@@ -563,6 +600,7 @@ static void run_ct(const void* x, const float* b, float* out, void* wt,
     if (half) {
       ha = *half;
       ha.gsc = half->gsc + s0;
+      if (odd) ha.cinv = half->cinv + (long)group * H;
     }
     const void* xg = static_cast<const char*>(x) + xel * o;
     const float* bg = b ? b + o : nullptr;
@@ -572,7 +610,7 @@ static void run_ct(const void* x, const float* b, float* out, void* wt,
       auto launch = [&](auto k) {
         allow_lds((const void*)k, smem);
         hipLaunchKernelGGL(k, dim3(nbr * ns), dim3(P::NT), smem, stream, xg,
-                           wt, tw, n, H, rp, nbr, cp, ha);
+                           wtg, tw, n, H, rp, nbr, cp, ha);
       };
       if (cm == 0) launch(ct_rows_fwd<P, 0, H16>);
       else if (cm == 1) launch(ct_rows_fwd<P, 1, H16>);
@@ -586,7 +624,7 @@ static void run_ct(const void* x, const float* b, float* out, void* wt,
       auto k = ct_cols<P, H16>;
       allow_lds((const void*)k, smemc);
       ktimer_begin(KC_TOEPLITZ_COLS, stream);
-      hipLaunchKernelGGL(k, dim3(nbc * ns), dim3(P::NTC), smemc, stream, wt,
+      hipLaunchKernelGGL(k, dim3(nbc * ns), dim3(P::NTC), smemc, stream, wtg,
                          spec, tw, n, H, rp, nbc,
                          partials ? partials + (long)s0 * nbc : nullptr, ha);
       TC_HIP(hipGetLastError());
@@ -598,7 +636,7 @@ static void run_ct(const void* x, const float* b, float* out, void* wt,
       ktimer_begin(KC_TOEPLITZ_ROWS_INV, stream);
       auto launch = [&](auto k) {
         allow_lds((const void*)k, smem);
-        hipLaunchKernelGGL(k, dim3(nbr * ns), dim3(P::NT), smem, stream, wt,
+        hipLaunchKernelGGL(k, dim3(nbr * ns), dim3(P::NT), smem, stream, wtg,
                            bg, og, tw, n, H, rp, nbr, cp, ha);
       };
       if (em == 0) launch(ct_rows_inv<P, 0, H16>);
@@ -610,6 +648,10 @@ static void run_ct(const void* x, const float* b, float* out, void* wt,
       const double extra = em == 1 || em == 2 ? 1.0 : (em == 3 ? 2.0 : 0.0);
       ktimer_end(KC_TOEPLITZ_ROWS_INV, stream, ns * (sbw + ib * (1.0 + extra)));
     }
+  }
+  if (dual) {
+    TC_HIP(hipEventRecord(ts->done, ts->aux));
+    TC_HIP(hipStreamWaitEvent(stream0, ts->done, 0));
   }
 }
 
@@ -635,10 +677,11 @@ static bool apply_ct(int M, const void* x, const float* b, float* out,
                      void* wt, const void* spec, int n, int H, int rp,
                      int nslices, int mode, double* partials,
                      hipStream_t stream, double img_b, double spec_b,
-                     const Psf::StepCommit* sc, const ToeplitzHalf* half) {
+                     const Psf::StepCommit* sc, const ToeplitzHalf* half,
+                     const ToeplitzStreams* ts) {
 #define TC_RUN_CT(PL)                                                        \
   run_ct<PL, H16>(x, b, out, wt, spec, n, H, rp, nslices, mode, partials,   \
-                  stream, img_b, spec_b, sc, half);                         \
+                  stream, img_b, spec_b, sc, half, ts);                     \
   return true
   switch (M) {
     case 750: TC_RUN_CT(Plan750);
@@ -656,9 +699,9 @@ bool toeplitz_apply_ct(int M, const float* x, const float* b, float* out,
                        float2* wt, const float* spec, int n, int H, int rp,
                        int nslices, int mode, double* partials,
                        hipStream_t stream, double img_b, double spec_b,
-                       const Psf::StepCommit* sc) {
+                       const Psf::StepCommit* sc, const ToeplitzStreams* ts) {
   return apply_ct<false>(M, x, b, out, wt, spec, n, H, rp, nslices, mode,
-                         partials, stream, img_b, spec_b, sc, nullptr);
+                         partials, stream, img_b, spec_b, sc, nullptr, ts);
 }
 
 bool toeplitz_apply_ct_half(int M, const void* x16, float* out, void* wt,
@@ -666,9 +709,10 @@ bool toeplitz_apply_ct_half(int M, const void* x16, float* out, void* wt,
                             int nslices, double* partials, hipStream_t stream,
                             double img_b, double spec_b,
                             const Psf::StepCommit* sc,
-                            const ToeplitzHalf& half) {
+                            const ToeplitzHalf& half,
+                            const ToeplitzStreams* ts) {
   return apply_ct<true>(M, x16, nullptr, out, wt, spec16, n, H, rp, nslices,
-                        0, partials, stream, img_b, spec_b, sc, &half);
+                        0, partials, stream, img_b, spec_b, sc, &half, ts);
 }
 
 }  // namespace tc
