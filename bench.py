@@ -49,6 +49,14 @@ WORKLOADS = {
         slices=64, npix=2048, nproj=1501, iterations=100, sigma=0.1, p=1.2,
         noise=0.005, config_index=3,
         label="64 slices 2048x2048, 1501 angles, 100 iterations (plain FP32)"),
+    # one forward + backward of the differentiable loss
+    # 0.5 ||P x - y||^2 + beta qGGMRF(x) (projector with the hand-written
+    # adjoint backward, free-boundary prior with its exact gradient kernel)
+    "autograd-8x512": dict(
+        kind="autograd", slices=8, npix=512, nproj=360, iterations=1,
+        sigma=0.1, p=1.2, beta=1e-3, noise=0.01, config_index=2,
+        label="differentiation: forward + backward of 0.5||Px-y||^2 + "
+              "1e-3 qGGMRF(x) on 8 slices 512x512, 360 angles x 512 rays"),
     # the mixed-precision comparison: sigma = 100 (the reference's default
     # smoothness 0.01), where the iteration converges and relative errors
     # between precisions are meaningful
@@ -160,7 +168,7 @@ def cpu_baseline(sino_host, ang, w, sample_slices, sample_iters):
         "kind": "port",
         "sample": (f"FP32 CPU restatement of mbir2 (4 Toeplitz products per "
                    f"attempt, scipy.fft + C/OpenMP): full setup and "
-                   f"{sample_iters} iterations on {sample_slices} of the "
+                   f"{sample_iters} iterations on {r['sample_slices']} of the "
                    f"{w['slices']} slices ({r['setup_s']:.1f} s setup, "
                    f"{r['iter_s']:.1f} s iterations), extrapolated linearly to "
                    f"{w['iterations']} iterations per slice"),
@@ -246,7 +254,24 @@ def main():
         log("copy calibration: 1 GiB read + 1 GiB written")
         del a, c
     uneven = w.get("kind") == "uneven"
-    if uneven:
+    autograd = w.get("kind") == "autograd"
+    if autograd:
+        from tomocam_amd import autograd as AG
+        img = torch.from_numpy(phantom.phantom_stack(
+            w["npix"], w["slices"], "ellipses", w["config_index"])).cuda()
+        ang = phantom.angles(w["nproj"])
+        sino = gpu.project(img, ang)
+        x = torch.zeros_like(img, requires_grad=True)
+        center = w["npix"] / 2.0
+
+        def step():
+            x.grad = None
+            loss = AG.mbir_loss(x, sino, ang, beta=w["beta"],
+                                sigma=w["sigma"], p=w["p"])
+            loss.backward()
+            torch.cuda.synchronize()
+            return None, {"loss": float(loss)}
+    elif uneven:
         stacks = make_uneven(torch, gpu, phantom, w)
         w["slices"] = sum(int(st[0].shape[0]) for st in stacks)
         log(f"{len(stacks)} stacks, {w['slices']} slices")
@@ -309,7 +334,7 @@ def main():
         }
 
     cpu = None
-    if (not args.no_cpu_baseline and not uneven
+    if (not args.no_cpu_baseline and not uneven and not autograd
             and int(os.environ.get("RANK", "0")) == 0):
         log("cpu baseline sample ...")
         cpu = cpu_baseline(sino.cpu().numpy(), ang, w, args.cpu_sample_slices,
@@ -346,6 +371,9 @@ def main():
             "slices": w["slices"], "iterations": w["iterations"],
             "sigma": w["sigma"],
         } if uneven else {
+            "workload": w["label"], "slices": w["slices"],
+            "npix": w["npix"], "nproj": w["nproj"], "beta": w["beta"],
+        } if autograd else {
             "workload": w["label"],
             "slices": w["slices"], "npix": w["npix"], "nproj": w["nproj"],
             "iterations": w["iterations"], "sigma": w["sigma"],

@@ -254,6 +254,17 @@ int tc_mbir(const float* d_sino, const float* d_x0, float* d_recon,
             float center, int num_iters, float sigma, float tol, float xtol,
             int verbose, double* h_info, void* stream);
 
+/* Real-data front end, device side: projections as read from the HDF5 file
+ * (tc_h5_read_projections in tomocam_h5.h), d_proj [nproj][nslices][ncols],
+ * to sinograms d_sino [nslices][nproj][ncols - drop_last_col].  Replaces
+ * the host transpose of tomocam::h5::Reader::read_sinogram
+ * (src/hdf5/reader.h:110-114) and DArray::dropcol (src/dist_array.h:
+ * 258-282), which test/mbir2.cpp:104-107 applies to even widths (the last
+ * column is dropped; the centre is kept). */
+int tc_projections_to_sinogram(const float* d_proj, float* d_sino, int nproj,
+                               int nslices, int ncols, int drop_last_col,
+                               void* stream);
+
 /* Uneven batching (no counterpart in the reference, which runs one stack
  * per call): nstacks independent MBIR reconstructions of different image
  * sizes, angle sets and slice counts in one call, each exactly tc_mbir2

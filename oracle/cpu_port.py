@@ -219,6 +219,7 @@ def sample_rate(sino: np.ndarray, angles, center: float, num_iters: int,
     """Times setup + ``sample_iters`` iterations on ``sample_slices`` slices
     and extrapolates to ``num_iters`` iterations, linearly per
     slice-iteration (slices are independent units of work)."""
+    sample_slices = max(1, min(sample_slices, sino.shape[0]))
     sub = np.ascontiguousarray(sino[:sample_slices])
     tm = {}
     mbir2(sub, angles, center, sample_iters, sigma, timing=tm)

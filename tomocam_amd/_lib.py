@@ -68,6 +68,8 @@ SIGNATURES = {
                 vp],
     "tc_mbir2_batch": [C.c_int, vp, vp, c_int_p, c_int_p, c_int_p, vp, vp,
                        C.c_int, C.c_float, C.c_int, c_double_p, vp],
+    "tc_projections_to_sinogram": [vp, vp, C.c_int, C.c_int, C.c_int,
+                                   C.c_int, vp],
     "tc_mbir2_host": [vp, vp, C.c_int, C.c_int, C.c_int, vp, C.c_float,
                       C.c_int, C.c_float, C.c_float, C.c_float, C.c_int],
     "tc_ktimer_enable": [C.c_int, C.c_int],

@@ -529,6 +529,17 @@ int tc_mbir2_batch(int nstacks, const float* const* d_sinos,
   });
 }
 
+int tc_projections_to_sinogram(const float* d_proj, float* d_sino, int nproj,
+                               int nslices, int ncols, int drop_last_col,
+                               void* stream) {
+  return guarded([&] {
+    TC_REQUIRE(d_proj && d_sino, "null argument");
+    TC_REQUIRE(d_proj != d_sino, "projections: in and out must differ");
+    tc::projections_to_sinogram(d_proj, d_sino, nproj, nslices, ncols,
+                                drop_last_col != 0, as_stream(stream));
+  });
+}
+
 int tc_mbir2_host(const float* h_sino, float* h_recon, int nslices, int nproj,
                   int nrays, const float* h_angles, float center,
                   int num_iters, float sigma, float tol, float xtol,

@@ -22,6 +22,10 @@ void postproc(const float* in, float* out, int S, int nin, int nout,
 void pad2d(const float* in, float* out, int S, int nin, int padding,
            hipStream_t stream);
 
+// [P][S][ncols] projections -> [S][P][ncols - drop_last] sinograms.
+void projections_to_sinogram(const float* proj, float* sino, int P, int S,
+                             int ncols, bool drop_last, hipStream_t stream);
+
 struct Mbir2Options {
   int num_iters = 100;
   float sigma = 100.f;

@@ -1,4 +1,5 @@
 // Pre/post-processing kernels and the device-resident mbir2 pipeline.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstdio>
@@ -216,6 +217,36 @@ Mbir2Result mbir2(const float* sino, const float* x0, float* recon, int S,
   postproc(bsol.as<float>(), recon, S, n2, N, stream);
   TC_HIP(hipStreamSynchronize(stream));
   return res;
+}
+
+// Real-data front end: projections as stored in the HDF5 file, [nproj]
+// [nslices][ncols], to sinograms [nslices][nproj][nout] (nout = ncols or
+// ncols - 1).  The reference transposes on the host (src/hdf5/reader.h:
+// 110-114) and drops the last column of even widths (DArray::dropcol,
+// src/dist_array.h:258-282; test/mbir2.cpp:104-107).
+__global__ void proj_to_sino_kernel(const float* __restrict__ in,
+                                    float* __restrict__ out, int P, int S,
+                                    int ncols, int nout, long total) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += stride) {
+    const long row = i / nout;
+    const int c = (int)(i - row * nout);
+    const int s = (int)(row / P);
+    const int p = (int)(row - (long)s * P);
+    out[i] = in[((long)p * S + s) * ncols + c];
+  }
+}
+
+void projections_to_sinogram(const float* proj, float* sino, int P, int S,
+                             int ncols, bool drop_last, hipStream_t stream) {
+  TC_REQUIRE(P >= 1 && S >= 1 && ncols >= 2, "projections: bad shape");
+  const int nout = ncols - (drop_last ? 1 : 0);
+  const long total = (long)S * P * nout;
+  const unsigned nblk = (unsigned)std::min<long>(cdiv(total, 256), 65536);
+  hipLaunchKernelGGL(proj_to_sino_kernel, dim3(nblk), dim3(256), 0, stream,
+                     proj, sino, P, S, ncols, nout, total);
+  TC_HIP(hipGetLastError());
 }
 
 // out = (1 + beta) a - beta b, i.e. a + beta (a - b)
