@@ -128,10 +128,11 @@ int tc_toeplitz_apply(tc_psf p, const float* d_x, float* d_out, int nslices,
 /* Autotuner of the Toeplitz passes (no counterpart in the reference): times
  * one application for slice-group sizes of half, once and twice the default
  * (a group's intermediate stays in the memory-side cache between the three
- * passes; the default keeps it near 144 MB), each on one stream and with
- * groups alternating over two streams, on scratch data, FP32 or the
+ * passes; the default keeps it near 144 MB) on scratch data, FP32 or the
  * mixed-precision passes, and keeps the fastest for this width, FFT length
- * and slice count for the rest of the process.  *group (optional) receives the choice, -1 if the shape was
+ * and slice count for the rest of the process.  (The environment variable
+ * TOMOCAM_HIP_TOEPLITZ_STREAMS=2 alternates slice groups over a second
+ * stream; it is not tuned.)  *group (optional) receives the choice, -1 if the shape was
  * tuned before, 0 if there is nothing to tune (no compile-time plan, or one
  * slice).  tc_mbir2 / tc_mbir2_mixed tune on first use unless the
  * environment sets TOMOCAM_HIP_AUTOTUNE=0.  Synchronises `stream`. */

@@ -86,9 +86,9 @@ class Psf {
   bool has_fused() const;
 
   // Autotuner: times one application of the compile-time passes for each
-  // candidate slice-group size (half, once and twice the 144 MB rule's), on
-  // one stream and alternating over two, on scratch data and keeps the
-  // fastest for this (n, M, precision, nslices) for the rest of the process.  Returns the chosen group size, -1 if this shape
+  // candidate slice-group size (half, once and twice the 144 MB rule's) on
+  // scratch data and keeps the fastest for this (n, M, precision, nslices)
+  // for the rest of the process.  Returns the chosen group size, -1 if this shape
   // was tuned before, 0 if there is nothing to tune (no compile-time plan,
   // or a single slice).
   int tune_groups(int nslices, bool half, hipStream_t stream);
