@@ -99,3 +99,37 @@ def test_batch_concurrent_error_names_lowest_stack(torch_cuda, monkeypatch):
     announce("mbir2 concurrent batch error path")
     with pytest.raises((ValueError, _lib.TomocamError), match="stack [13]"):
         gpu.mbir_batch(stacks, 2, 10.0)
+
+
+def test_concurrent_host_threads(torch_cuda):
+    """Two host threads calling tc_mbir2 at once (each on its own stream)
+    get the same results as one after the other: every thread has NUFFT
+    plans of its own."""
+    import threading
+    from tomocam_amd import gpu
+    st = _stacks(torch_cuda)
+    jobs = [(s, a, c) for s, a, c, _ in st[:2]]
+    seq = [gpu.mbir(s, a, c, 6, 10.0)[0] for s, a, c in jobs]
+    out = [None, None]
+    errs = []
+
+    def run(k):
+        try:
+            s, a, c = jobs[k]
+            strm = torch.cuda.Stream()
+            with torch.cuda.stream(strm):
+                r, _ = gpu.mbir(s, a, c, 6, 10.0)
+            strm.synchronize()
+            out[k] = r
+        except Exception as e:  # reported below
+            errs.append(e)
+
+    announce("two host threads calling mbir2")
+    th = [threading.Thread(target=run, args=(k,)) for k in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errs, errs
+    for k in range(2):
+        assert torch.equal(out[k], seq[k]), k

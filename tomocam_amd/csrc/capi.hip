@@ -492,11 +492,10 @@ int tc_mbir2_batch(int nstacks, const float* const* d_sinos,
       }
       stop = true;
     };
-    auto work = [&](int w) {
+    auto work = [&](int /*worker*/) {
       int k = -1;
       try {
         TC_HIP(hipSetDevice(dev));
-        tc::set_plan_slot(w + 1);
         hipStream_t s = nullptr;
         TC_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         struct StreamFree {

@@ -79,10 +79,10 @@ class RayDft {
 };
 
 // Cached NUFFT / ray-DFT plans (per device and geometry); live until exit.
-// NUFFT plans are also per plan slot of the calling thread (see
-// set_plan_slot): concurrent threads must use different slots.
+// NUFFT plans hold scratch, so they are also per plan slot: each host thread
+// holds its own slot while it lives, and concurrent threads never share a
+// plan.
 PolarNufft& nufft_plan(const float* h_angles, int nproj, int npix, int nmodes);
-void set_plan_slot(int slot);
 RayDft& ray_dft_plan(int N, int sign);
 
 // P x (src/forward.cpp:33-52): img [S][N][N] -> sino [S][P][N].

@@ -856,10 +856,36 @@ void RayDft::apply(const void* in, bool in_real, void* out, bool out_real,
 // ---------------------------------------------------------------------------
 
 // A PolarNufft keeps scratch (the fine grid) between calls, so threads that
-// run concurrently use plans of their own: the cache key holds the calling
-// thread's plan slot (0 unless set_plan_slot gave it another).
-static thread_local int t_plan_slot = 0;
-void set_plan_slot(int slot) { t_plan_slot = slot; }
+// run concurrently use plans of their own: the cache key holds a plan slot,
+// taken by each thread on its first plan request from a pool (lowest free
+// number) and given back when the thread ends.  Threads that follow one
+// another, such as the batch workers of successive calls, reuse the slots
+// and with them the cached plans.
+static std::mutex g_slot_mu;
+static std::vector<char> g_slot_used;
+
+struct PlanSlot {
+  int id = -1;
+  ~PlanSlot() {
+    if (id >= 0) {
+      std::lock_guard<std::mutex> lk(g_slot_mu);
+      g_slot_used[id] = 0;
+    }
+  }
+};
+static thread_local PlanSlot t_slot;
+
+static int plan_slot() {
+  if (t_slot.id < 0) {
+    std::lock_guard<std::mutex> lk(g_slot_mu);
+    int i = 0;
+    while (i < (int)g_slot_used.size() && g_slot_used[i]) ++i;
+    if (i == (int)g_slot_used.size()) g_slot_used.push_back(0);
+    g_slot_used[i] = 1;
+    t_slot.id = i;
+  }
+  return t_slot.id;
+}
 
 PolarNufft& nufft_plan(const float* h_angles, int nproj, int npix,
                        int nmodes) {
@@ -869,7 +895,7 @@ PolarNufft& nufft_plan(const float* h_angles, int nproj, int npix,
   int dev = 0;
   TC_HIP(hipGetDevice(&dev));
   std::vector<float> ang(h_angles, h_angles + nproj);
-  const int slot = t_plan_slot;
+  const int slot = plan_slot();
   Key key(slot, dev, nproj, npix, nmodes, ang);
   std::lock_guard<std::mutex> lk(mu);
   auto it = cache.find(key);
