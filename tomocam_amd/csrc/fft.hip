@@ -215,6 +215,66 @@ void fft_lines(const LineFFTArgs& a, int dir, hipStream_t stream) {
   throw Error(TC_ERR_INTERNAL, "fft_lines: no kernel for configuration");
 }
 
+// Column transforms restricted to the columns [0, lo) and [cols - hi, cols)
+// (the others are not needed, or are zero, in the caller).
+static void fft_cols_ranges(float2* data, int nbatch, int rows, int cols,
+                            long row_pitch, long slice_pitch, int lo, int hi,
+                            int dir, hipStream_t stream) {
+  if (lo + hi >= cols) {
+    lo = cols;
+    hi = 0;
+  }
+  LineFFTArgs a;
+  a.L = rows;
+  a.nbatch = nbatch;
+  a.bstride = slice_pitch;
+  a.lstride = 1;
+  a.estride = row_pitch;
+  a.nin = rows;
+  a.nout = rows;
+  a.colmajor = true;
+  if (lo > 0) {
+    a.in = a.out = data;
+    a.lines_per_batch = lo;
+    fft_lines(a, dir, stream);
+  }
+  if (hi > 0) {
+    a.in = a.out = data + (cols - hi);
+    a.lines_per_batch = hi;
+    fft_lines(a, dir, stream);
+  }
+}
+
+static void fft_rows_all(float2* data, int nbatch, int rows, int cols,
+                         long row_pitch, long slice_pitch, int dir,
+                         hipStream_t stream) {
+  LineFFTArgs a;
+  a.in = data;
+  a.out = data;
+  a.L = cols;
+  a.lines_per_batch = rows;
+  a.nbatch = nbatch;
+  a.bstride = slice_pitch;
+  a.lstride = row_pitch;
+  a.estride = 1;
+  a.nin = cols;
+  a.nout = cols;
+  a.colmajor = false;
+  fft_lines(a, dir, stream);
+}
+
+void fft2d_pruned(float2* data, int nbatch, int n, int lo, int hi, int dir,
+                  bool rows_first, hipStream_t stream) {
+  const long pitch = n, slice = (long)n * n;
+  if (rows_first) {
+    fft_rows_all(data, nbatch, n, n, pitch, slice, dir, stream);
+    fft_cols_ranges(data, nbatch, n, n, pitch, slice, lo, hi, dir, stream);
+  } else {
+    fft_cols_ranges(data, nbatch, n, n, pitch, slice, lo, hi, dir, stream);
+    fft_rows_all(data, nbatch, n, n, pitch, slice, dir, stream);
+  }
+}
+
 void fft2d(float2* data, int nbatch, int rows, int cols, long row_pitch,
            long slice_pitch, int dir, hipStream_t stream) {
   LineFFTArgs a;

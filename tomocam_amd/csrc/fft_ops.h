@@ -28,4 +28,12 @@ void fft_lines(const LineFFTArgs& a, int dir, hipStream_t stream);
 void fft2d(float2* data, int nbatch, int rows, int cols, long row_pitch,
            long slice_pitch, int dir, hipStream_t stream);
 
+// The same on contiguous [n][n] arrays when only the columns [0, lo) and
+// [n - hi, n) matter: rows_first transforms every row, then only those
+// columns (the other columns of the result are left unfinished); otherwise
+// those columns are transformed first (the others must be zero on input)
+// and then every row.
+void fft2d_pruned(float2* data, int nbatch, int n, int lo, int hi, int dir,
+                  bool rows_first, hipStream_t stream);
+
 }  // namespace tc

@@ -698,7 +698,8 @@ void PolarNufft::type1_real(const float2* c, float* out, int S, float scale,
                          scratch.as<float2>());
       TC_HIP(hipGetLastError());
     }
-    fft2d(fg, nb, nf_, nf_, nf_, (long)nf_ * nf_, +1, stream);
+    // only the columns of modes [-nm/2, (nm+1)/2) are read back
+    fft2d_pruned(fg, nb, nf_, (nm_ + 1) / 2, nm_ / 2, +1, true, stream);
     const long tot = (long)nb * nm_ * nm_;
     hipLaunchKernelGGL(deconv_real_kernel, dim3(cdiv(tot, 256)), dim3(256), 0,
                        stream, fg, out + (long)s0 * nm_ * nm_, corr_.as<float>(),
@@ -721,7 +722,8 @@ void PolarNufft::type2_real(const float* f, float2* c, int S,
                        stream, f + (long)s0 * nm_ * nm_, fg, corr_.as<float>(),
                        nm_, nf_, nb);
     TC_HIP(hipGetLastError());
-    fft2d(fg, nb, nf_, nf_, nf_, (long)nf_ * nf_, -1, stream);
+    // only the columns of modes [-nm/2, (nm+1)/2) are non-zero
+    fft2d_pruned(fg, nb, nf_, (nm_ + 1) / 2, nm_ / 2, -1, false, stream);
     InterpArgs a;
     a.fine = fg;
     a.c = c + (long)s0 * P_ * N_;

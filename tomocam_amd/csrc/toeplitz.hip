@@ -75,7 +75,8 @@ Psf::Psf(const float* d_psf, int n, hipStream_t stream) : n_(n) {
   hipLaunchKernelGGL(psf_place_kernel, dim3(cdiv(tot, 256)), dim3(256), 0,
                      stream, d_psf, c.as<float2>(), n, M_);
   TC_HIP(hipGetLastError());
-  fft2d(c.as<float2>(), 1, M_, M_, M_, (long)M_ * M_, -1, stream);
+  // the spectrum keeps columns k1 < H only (real, even PSF)
+  fft2d_pruned(c.as<float2>(), 1, M_, H_, 0, -1, true, stream);
   spec_.alloc(sizeof(float) * (size_t)M_ * H_);
   const double scale = 1.0 / ((double)n * n * n * (double)M_ * M_);
   long tot2 = (long)M_ * H_;
