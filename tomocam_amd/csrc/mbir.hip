@@ -550,6 +550,25 @@ __global__ void reduce_strided_kernel(const double* __restrict__ p, long count,
   if (threadIdx.x == 0) out[k] = tot;
 }
 
+// The line search's sums in one launch: out[k] = sum_i q[i*nq + k] for
+// k < nq (the gradient kernel's per-workgroup dots), out[nq] = sum_i t[i]
+// (the Toeplitz column pass's g.Tg partials).  Same order per value as
+// reduce_strided_kernel.
+__global__ void reduce_dots_kernel(const double* __restrict__ q, long qcount,
+                                   int nq, const double* __restrict__ t,
+                                   long tcount, double* out) {
+  __shared__ double red[16];
+  const int k = blockIdx.x;
+  const double* p = k < nq ? q + k : t;
+  const long count = k < nq ? qcount : tcount;
+  const int stride = k < nq ? nq : 1;
+  double acc = 0.0;
+  for (long i = threadIdx.x; i < count; i += blockDim.x)
+    acc += p[i * stride];
+  double tot = block_sum(acc, red);
+  if (threadIdx.x == 0) out[k] = tot;
+}
+
 // ---------------------------------------------------------------------------
 // per-slice scales of the FP16 gradient (mixed precision)
 // ---------------------------------------------------------------------------
@@ -764,11 +783,9 @@ Run2Stats run2_device(Psf& psf, const float* x0, const float* b, float* out,
       if (opt.mixed) std::swap(gsc, gsc_next);
       if (!fused) psf.apply(g, nullptr, Tg, S, 0, tparts, stream);
       ++st.toeplitz_applies;
-      hipLaunchKernelGGL(reduce_strided_kernel, dim3(NDOT), dim3(1024), 0,
-                         stream, qparts, (long)qblk, NDOT,
+      hipLaunchKernelGGL(reduce_dots_kernel, dim3(NDOT + 1), dim3(1024), 0,
+                         stream, qparts, (long)qblk, NDOT, tparts, (long)ntp,
                          dots_dev.as<double>());
-      hipLaunchKernelGGL(reduce_strided_kernel, dim3(1), dim3(1024), 0, stream,
-                         tparts, (long)ntp, 1, dots_dev.as<double>() + NDOT);
       TC_HIP(hipGetLastError());
       TC_HIP(hipMemcpyAsync(dots_h, dots_dev.as<double>(), sizeof(double) * 8,
                             hipMemcpyDeviceToHost, stream));
