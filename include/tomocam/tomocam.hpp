@@ -372,6 +372,32 @@ DArray<T> mbir2(std::optional<DArray<T>> guess, const DArray<T>& sino,
   return out;
 }
 
+// Mixed-precision form of mbir2 (no counterpart in the reference): FP16
+// gradient and Toeplitz intermediates, FP32 FFTs and state, FP64 dots, T x
+// refreshed in FP32 every `refresh_every` iterations.  The padded width must
+// have a compile-time FFT plan (std::invalid_argument otherwise).
+template <typename T>
+DArray<T> mbir2_mixed(std::optional<DArray<T>> guess, const DArray<T>& sino,
+                      std::vector<T> angles, T center, int num_iters, T sigma,
+                      int refresh_every = 10) {
+  detail::require_float<T>();
+  const int S = sino.nslices(), P = sino.nrows(), N = sino.ncols();
+  detail::Dev d_sino(sino.begin(), sino.size());
+  std::unique_ptr<detail::Dev> d_guess;
+  if (guess)
+    d_guess.reset(new detail::Dev(guess->begin(), guess->size()));
+  DArray<T> out(dim3_t(S, N, N));
+  detail::Dev d_out(out.size());
+  detail::check(tc_mbir2_mixed(d_sino.get(),
+                               d_guess ? d_guess->get() : nullptr,
+                               d_out.get(), S, P, N, angles.data(), center,
+                               num_iters, sigma, refresh_every, 0, 0, nullptr,
+                               nullptr),
+                "mbir2_mixed");
+  d_out.to_host(out.begin());
+  return out;
+}
+
 // src/mbir.cpp:38-119 (the NUFFT normal-operator variant; x0 required)
 template <typename T>
 DArray<T> mbir(DArray<T>& x0, DArray<T>& sino, std::vector<T> angles,

@@ -80,6 +80,42 @@ int main(int argc, char** argv) {
                                      10.0f, 1e-4f, 1e-4f);
     dump(dir, "mbir.f32", rec.begin(), rec.size());
 
+    // the mixed-precision form needs a compile-time FFT length: width 32
+    // pads to 44 (FFT length 90), which has none
+    bool mixed_threw = false;
+    try {
+      tomocam::mbir2_mixed<float>(std::nullopt, sino, ang, N / 2.0f, 2, 10.0f);
+    } catch (const std::invalid_argument&) {
+      mixed_threw = true;
+    }
+    std::printf("mixed without plan threw: %d\n", mixed_threw ? 1 : 0);
+    if (!mixed_threw) return 1;
+    // and agrees with mbir2 at width 256 (padded 362, FFT length 750)
+    {
+      const int N2 = 256, P2 = 90;
+      tomocam::DArray<float> img2(tomocam::dim3_t(1, N2, N2));
+      for (int r = 0; r < N2; ++r)
+        for (int c = 0; c < N2; ++c) {
+          const float dx = (c - N2 / 2) / (float)N2, dy = (r - N2 / 2) / (float)N2;
+          img2[(size_t)r * N2 + c] = dx * dx + 2 * dy * dy < 0.1f ? 1.f : 0.f;
+        }
+      std::vector<float> ang2(P2);
+      for (int p = 0; p < P2; ++p) ang2[p] = (float)(p * M_PI / P2);
+      auto s2 = tomocam::project(img2, ang2);
+      auto r32 = tomocam::mbir2<float>(std::nullopt, s2, ang2, N2 / 2.0f, 20,
+                                       100.0f, 0.f, 0.f);
+      auto r16 = tomocam::mbir2_mixed<float>(std::nullopt, s2, ang2,
+                                             N2 / 2.0f, 20, 100.0f);
+      double num = 0.0, den = 0.0;
+      for (size_t i = 0; i < r32.size(); ++i) {
+        num += ((double)r16[i] - r32[i]) * ((double)r16[i] - r32[i]);
+        den += (double)r32[i] * r32[i];
+      }
+      const double rel = std::sqrt(num / den);
+      std::printf("mixed vs fp32 relL2 %.3e\n", rel);
+      if (!(rel < 1e-4)) return 1;
+    }
+
     // an all-zero sinogram throws (no guess)
     tomocam::DArray<float> zero(sino.dims());
     zero.init(0.f);

@@ -181,3 +181,24 @@ def test_mbir2_mixed_argument_checks(torch_cuda):
         gpu.mbir(sino, ang, 50.0, 2, 100.0, precision="bf16")
     with pytest.raises(ValueError):
         gpu.mbir(sino, ang, 50.0, 2, 100.0, lipschitz_iters=5)
+
+
+def test_python_entry_points_precision_keyword(torch_cuda):
+    """tomocam.MBIR / recon / cTomocam.mbir take precision="mixed" as a
+    keyword (the positional reference signature is unchanged)."""
+    import tomocam_amd as tomocam
+    from tomocam_amd import cTomocam
+    N, P, S = 256, 90, 1
+    ang, sino = _sino(N, P, S, 14)
+    s32 = sino.astype(np.float32)
+    announce("MBIR precision keyword")
+    plain = tomocam.MBIR(s32, ang, N / 2, 20, 0.01)
+    mixed = tomocam.MBIR(s32, ang, N / 2, 20, 0.01, precision="mixed")
+    assert rel_l2(mixed, plain) < 1e-4
+    again = tomocam.recon(s32, ang, N / 2, num_iters=20, precision="mixed")
+    assert np.array_equal(again, mixed)
+    direct = cTomocam.mbir(s32, ang, N / 2, 20, 100.0, 1e-4, 1e-4,
+                           precision="mixed")
+    assert np.array_equal(direct, mixed)
+    with pytest.raises(ValueError):
+        tomocam.MBIR(s32, ang, N / 2, 2, 0.01, precision="fp8")

@@ -63,8 +63,11 @@ def backproject(sino, angs, cen) -> np.ndarray:
     return gpu.postproc(img, nrays).cpu().numpy()
 
 
-def mbir(sino, angles, center, num_iters, sigma, tol, xtol) -> np.ndarray:
-    """tomocam::mbir2 (src/pyTomocam.cpp:116-134): [S, P, N] -> [S, N, N]."""
+def mbir(sino, angles, center, num_iters, sigma, tol, xtol, *,
+         precision="fp32") -> np.ndarray:
+    """tomocam::mbir2 (src/pyTomocam.cpp:116-134): [S, P, N] -> [S, N, N].
+    ``precision="mixed"`` (keyword only, no counterpart in the reference)
+    runs the mixed-precision form."""
     from . import gpu
     torch = _torch()
     s = _stack3(sino)
@@ -74,5 +77,5 @@ def mbir(sino, angles, center, num_iters, sigma, tol, xtol) -> np.ndarray:
             f"{ang.size} angles for a sinogram with {s.shape[1]} projections")
     out, _ = gpu.mbir(torch.from_numpy(s).cuda(), ang, float(center),
                       int(num_iters), float(sigma), float(tol), float(xtol),
-                      verbose=True)
+                      verbose=True, precision=precision)
     return out.cpu().numpy()

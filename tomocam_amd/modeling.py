@@ -19,13 +19,18 @@ from . import cTomocam
 
 
 def MBIR(sinogram, angles, center, num_iters=100, smoothness=0.01, tol=1.0E-04,
-         xtol=1.0E-04):
+         xtol=1.0E-04, *, precision="fp32"):
     """Model-based iterative reconstruction (Toeplitz-gradient form).
 
     sinogram: [S, P, N] (or [P, N]) projection data; angles: [P] radians;
     center: rotation centre in detector pixels; smoothness > 0 (sigma =
     1/smoothness).  Returns the [S, N, N] reconstruction (in units of the
     sinogram normalised by its maximum, like the reference).
+
+    ``precision="mixed"`` (keyword only; the reference is FP32 throughout)
+    selects the mixed-precision form: FP16 gradient and Toeplitz
+    intermediates, FP32 FFTs and state, FP64 dots, T x refreshed in FP32
+    every 10 iterations.
     """
     if sinogram.dtype != np.float32:
         sinogram = sinogram.astype(np.float32)
@@ -34,14 +39,17 @@ def MBIR(sinogram, angles, center, num_iters=100, smoothness=0.01, tol=1.0E-04,
     if smoothness <= 0:
         raise ValueError('smoothness value must be greater than 0')
     sigma = 1. / smoothness
+    if precision != "fp32":
+        return cTomocam.mbir(sinogram, angles, center, num_iters, sigma, tol,
+                             xtol, precision=precision)
     return cTomocam.mbir(sinogram, angles, center, num_iters, sigma, tol, xtol)
 
 
 def recon(sinogram, angles, center, num_iters=100, smoothness=0.01,
-          tol=1.0E-04, xtol=1.0E-04):
+          tol=1.0E-04, xtol=1.0E-04, *, precision="fp32"):
     """README form ``recon(sino, theta, center)``; same as :func:`MBIR`."""
     return MBIR(sinogram, np.asarray(angles), center, num_iters, smoothness,
-                tol, xtol)
+                tol, xtol, precision=precision)
 
 
 def radon(volume, angles, center=None, over_sample=None):
