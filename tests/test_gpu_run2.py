@@ -167,3 +167,30 @@ def test_run2_fused_commit_with_retries(torch_cuda, n, S, mult, retries):
     assert stats2["backtracks"] == retries
     assert t.equal(out, out2)
     np.testing.assert_array_equal(err, err2)
+
+
+def test_run2_slice_groups_and_two_streams(torch_cuda, monkeypatch):
+    """The compile-time passes in several slice groups (5 slices in groups of
+    2: 2 + 2 + 1), on one stream and alternating over two, against one
+    group and against the FP64 oracle (padded width 362, FFT length 750)."""
+    torch = torch_cuda
+    ang = PH.angles(60).astype(np.float64)
+    img = np.stack([PH.random_ellipses(256, 500 + s) for s in range(5)])
+    sino = O.project(img, ang)
+    st = O.mbir2_setup(sino, ang, 128.0)
+    iters, sigma = 15, 100.0
+    T = lambda v: O.toeplitz(v, st.psf)
+    reg = lambda v: O.qggmrf_grad(v, sigma, 1.2)
+    ref = O.run2_linear(st.x0, T, st.sinoT, reg, st.sino_norm, iters, st.step)
+    monkeypatch.setenv("TOMOCAM_HIP_AUTOTUNE", "0")
+    outs = {}
+    for groups, streams in (("0", "1"), ("2", "1"), ("2", "2")):
+        monkeypatch.setenv("TOMOCAM_HIP_TOEPLITZ_SLICES", groups)
+        monkeypatch.setenv("TOMOCAM_HIP_TOEPLITZ_STREAMS", streams)
+        announce(f"run2 slice groups {groups} streams {streams}")
+        out = gpu_run2(torch, st, st.x0, iters, st.step, sigma)[0]
+        outs[(groups, streams)] = out.cpu().numpy()
+        assert rel_l2(outs[(groups, streams)], ref) < 1e-4
+    one = outs[("0", "1")]
+    assert rel_l2(outs[("2", "1")], one) < 1e-6
+    assert rel_l2(outs[("2", "2")], one) < 1e-6
