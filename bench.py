@@ -77,6 +77,31 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles",
                             "mbir_fp32_32x1024_hbm_traffic.json")
 
 
+PEAK_VALU_TFLOPS = 78.6  # FP32 vector FMA without packed math (2 flop/lane/clk)
+
+
+def valu_roofline(kd, n, L, half=False):
+    """The column pass against the FP32 vector-issue peak.  Its time goes to
+    FFT arithmetic (VALU bound), so next to the HBM roofline it is also
+    rated by algorithmic flops: 5 L log2(L) per length-L complex transform,
+    two per column (forward and inverse), plus the 2L of the real spectrum
+    multiply; H = L/2 + 1 columns per slice.  The slices per launch follow
+    from the launch's algorithmic bytes (ns * 2 * 8nH + 4HL in FP32,
+    ns * 2 * 4nH + 2HL with FP16 intermediates)."""
+    import math
+    H = L // 2 + 1
+    if half:
+        ns = (kd["bytes_per_launch"] - 2.0 * H * L) / (8.0 * n * H)
+    else:
+        ns = (kd["bytes_per_launch"] - 4.0 * H * L) / (16.0 * n * H)
+    flops = ns * H * (2 * 5.0 * L * math.log2(L) + 2.0 * L)
+    ach = flops / (kd["avg_ms"] * 1e-3) / 1e12
+    return {"kernel": "toeplitz_cols", "bound": "valu", "achieved": ach,
+            "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
+            "frac": ach / PEAK_VALU_TFLOPS, "flops_per_launch": flops,
+            "slices_per_launch": ns, "fft_length": L}
+
+
 def measured_traffic(workload, kernel_class):
     if workload != "plain-fp32-32x1024" or not os.path.exists(TRAFFIC_FILE):
         return None
@@ -332,6 +357,15 @@ def main():
             "bytes_per_launch": kd["bytes_per_launch"],
             "avg_launch_ms": kd["avg_ms"],
         }
+    valu = None
+    if ("toeplitz_cols" in kernels and not uneven and not autograd
+            and info and info.get("padded")):
+        import ctypes as C
+        n = int(info["padded"])
+        L = C.c_int()
+        _lib.call("tc_toeplitz_fft_length", n, C.byref(L))
+        valu = valu_roofline(kernels["toeplitz_cols"], n, L.value,
+                             half=w.get("precision") == "mixed")
 
     cpu = None
     if (not args.no_cpu_baseline and not uneven and not autograd
@@ -381,6 +415,7 @@ def main():
             "accepted_iterations": int(info["iterations"]),
         }),
         "roofline": roofline,
+        "valu_roofline": valu,
         "kernels": kernels,
         "cpu_baseline": cpu,
         "mixed_precision": mixed,

@@ -118,6 +118,12 @@ int tc_psf_destroy(tc_psf p);
 /* Image width n and FFT size M used by the convolution. */
 int tc_psf_info(tc_psf p, int* n, int* fft_size);
 
+/* The FFT size M a PSF of width n would use (M >= 2n - 1; the reference
+ * uses 2n - 1, src/toeplitz.h:57-60): the smallest length with compile-time
+ * passes when within 10 % of the smallest even 2,3,5,7-smooth length, else
+ * that.  No GPU needed. */
+int tc_toeplitz_fft_length(int n, int* fft_size);
+
 /* PointSpreadFunction::convolve (src/toeplitz.h:95-101):
  * d_out = T d_x = sum_q x[q] PSF(a - q) / n^3 per slice.  If h_xTx is not
  * NULL it receives x . T x (FP64, computed in the spectral domain) and the

@@ -206,6 +206,13 @@ int tc_toeplitz_apply(tc_psf p, const float* d_x, float* d_out, int nslices,
   });
 }
 
+int tc_toeplitz_fft_length(int n, int* fft_size) {
+  return guarded([&] {
+    TC_REQUIRE(n >= 2 && fft_size, "Toeplitz: bad width");
+    *fft_size = tc::toeplitz_fft_length(n);
+  });
+}
+
 int tc_psf_tune(tc_psf p, int nslices, int mixed, int* group, void* stream) {
   return guarded([&] {
     tc::Psf& P = get_psf(p);

@@ -123,6 +123,9 @@ struct ToeplitzHalf {
   float pconst;      // 1 / (2^e wsc^2): spectral dot back to input units
 };
 
+// FFT length M (>= 2n - 1) of the Toeplitz product for an n x n image.
+int toeplitz_fft_length(int n);
+
 // Number of partial sums Psf::apply writes for `nslices` slices.
 int toeplitz_num_partials(const Psf& p, int nslices);
 

@@ -20,7 +20,7 @@ extern const int kToeplitzCtCount;
 // FFT length for an n x n Toeplitz product: any even length >= 2n - 1 gives
 // the same result; take the smallest compile-time plan size >= 2n - 1 when
 // it is within 10 % of the smallest even 2,3,5,7-smooth length, else that.
-static int toeplitz_fft_length(int n) {
+int toeplitz_fft_length(int n) {
   const int m = 2 * n - 1;
   const int smooth = fft_next_smooth(m, true);
   for (int i = 0; i < kToeplitzCtCount; ++i) {
