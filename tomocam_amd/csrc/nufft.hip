@@ -490,7 +490,10 @@ __global__ void __launch_bounds__(256) interp_kernel(InterpArgs a) {
   for (int s = 0; s < a.nsl; ++s) {
     const float2* g = a.fine + (long)s * a.nf * a.nf;
     float2 acc = make_float2(0.f, 0.f);
-#pragma unroll
+    // 4 rows (32 gathers) in flight: fully unrolled, 64 gathers held 174
+    // VGPRs and two waves per SIMD; 4 measured 17 % faster (projection of
+    // 32 x 1024^2) than 8, and faster than 1 or 2
+#pragma unroll 4
     for (int r = 0; r < NUFFT_W; ++r) {
       const float2* row = g + (long)iy[r] * a.nf;
       float2 ra = make_float2(0.f, 0.f);
