@@ -38,6 +38,13 @@ struct Handle {
 
 template <class F>
 int guarded(F&& f) {
+  // errors are reported through the status and tc_h5_last_error(), not
+  // HDF5's automatic stderr dump
+  static const bool quiet = [] {
+    H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+    return true;
+  }();
+  (void)quiet;
   try {
     f();
     return 0;
