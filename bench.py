@@ -31,30 +31,30 @@ if ROOT not in sys.path:
 WORKLOADS = {
     "plain-fp32-32x1024": dict(
         slices=32, npix=1024, nproj=720, iterations=100, sigma=0.1, p=1.2,
-        noise=0.01, config_index=1,
+        noise=0.01, seed_group=1,
         label="plain FP32 Toeplitz MBIR: 32 slices 1024x1024, 720 angles x "
               "1024 rays, 100 iterations, qGGMRF sigma=0.1 p=1.2"),
     "cpu-path-4x256": dict(
         slices=4, npix=256, nproj=180, iterations=50, sigma=0.1, p=1.2,
-        noise=0.01, config_index=0, kind="shepp",
+        noise=0.01, seed_group=0, kind="shepp",
         label="4 slices 256x256 Shepp-Logan, 180 angles, 50 iterations"),
     "uneven-200": dict(
         kind="uneven", stacks=200, sizes=(256, 512, 768, 1024, 1536),
         pmin=180, pmax=1500, smin=1, smax=16, iterations=50, sigma=0.1,
-        p=1.2, noise=0.01, config_index=4,
+        p=1.2, noise=0.01, seed_group=4,
         label="uneven batching: 200 seeded stacks, image size in "
               "{256,512,768,1024,1536}, angles U{180..1500}, rays = size, "
               "1-16 slices each, 50 iterations, qGGMRF sigma=0.1 p=1.2"),
     "mixed-size-64x2048": dict(
         slices=64, npix=2048, nproj=1501, iterations=100, sigma=0.1, p=1.2,
-        noise=0.005, config_index=3,
+        noise=0.005, seed_group=3,
         label="64 slices 2048x2048, 1501 angles, 100 iterations (plain FP32)"),
     # one forward + backward of the differentiable loss
     # 0.5 ||P x - y||^2 + beta qGGMRF(x) (projector with the hand-written
     # adjoint backward, free-boundary prior with its exact gradient kernel)
     "autograd-8x512": dict(
         kind="autograd", slices=8, npix=512, nproj=360, iterations=1,
-        sigma=0.1, p=1.2, beta=1e-3, noise=0.01, config_index=2,
+        sigma=0.1, p=1.2, beta=1e-3, noise=0.01, seed_group=2,
         label="differentiation: forward + backward of 0.5||Px-y||^2 + "
               "1e-3 qGGMRF(x) on 8 slices 512x512, 360 angles x 512 rays"),
     # the mixed-precision comparison: sigma = 100 (the reference's default
@@ -62,7 +62,7 @@ WORKLOADS = {
     # between precisions are meaningful
     "mixed-prec-64x2048": dict(
         slices=64, npix=2048, nproj=1501, iterations=100, sigma=100.0, p=1.2,
-        noise=0.005, config_index=3, precision="mixed",
+        noise=0.005, seed_group=3, precision="mixed",
         label="mixed precision (FP16 gradient and Toeplitz intermediates, "
               "FP32 FFTs and state, FP64 dots, FP32 refresh every 10): 64 "
               "slices 2048x2048, 1501 angles x 2048 rays, 100 iterations, "
@@ -121,16 +121,16 @@ def make_sinogram(torch, gpu, phantom, w):
     if kind == "ellipses" and w["npix"] * w["npix"] * w["slices"] > (1 << 26):
         # large stacks: the same phantoms (phantom_stack's seeds) rasterised
         # on the GPU
-        seeds = [1000 * w["config_index"] + s for s in range(w["slices"])]
+        seeds = [1000 * w["seed_group"] + s for s in range(w["slices"])]
         img = _ellipse_stack_torch(torch, w["npix"], seeds)
     else:
         img = torch.from_numpy(phantom.phantom_stack(
-            w["npix"], w["slices"], kind, w["config_index"])).cuda()
+            w["npix"], w["slices"], kind, w["seed_group"])).cuda()
     sino = gpu.project(img, ang)
     del img
     torch.cuda.synchronize()
     g = torch.Generator(device="cuda")
-    g.manual_seed(1000 * w["config_index"] + 7)
+    g.manual_seed(1000 * w["seed_group"] + 7)
     sd = w["noise"] * float(sino.max())
     sino = sino + sd * torch.randn(sino.shape, generator=g, device="cuda")
     return sino.contiguous(), ang
@@ -162,13 +162,13 @@ def _ellipse_stack_torch(torch, n, seeds):
 def make_uneven(torch, gpu, phantom, w):
     """The uneven-batching workload: stack sizes drawn from a seeded rng;
     slice phantoms seeded 1000*config + running slice index."""
-    rng = np.random.default_rng(1000 * w["config_index"])
+    rng = np.random.default_rng(1000 * w["seed_group"])
     stacks, sl = [], 0
     for k in range(w["stacks"]):
         n = int(rng.choice(w["sizes"]))
         P = int(rng.integers(w["pmin"], w["pmax"] + 1))
         S = int(rng.integers(w["smin"], w["smax"] + 1))
-        seeds = [1000 * w["config_index"] + sl + i for i in range(S)]
+        seeds = [1000 * w["seed_group"] + sl + i for i in range(S)]
         sl += S
         img = _ellipse_stack_torch(torch, n, seeds)
         ang = phantom.angles(P)
@@ -283,7 +283,7 @@ def main():
     if autograd:
         from tomocam_amd import autograd as AG
         img = torch.from_numpy(phantom.phantom_stack(
-            w["npix"], w["slices"], "ellipses", w["config_index"])).cuda()
+            w["npix"], w["slices"], "ellipses", w["seed_group"])).cuda()
         ang = phantom.angles(w["nproj"])
         sino = gpu.project(img, ang)
         x = torch.zeros_like(img, requires_grad=True)

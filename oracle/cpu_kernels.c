@@ -1,7 +1,7 @@
 /*
  * CPU kernels of the FP32 CPU restatement of mbir2 (the benchmark's CPU
- * baseline).  TEST / BASELINE INFRASTRUCTURE ONLY: nothing in the product
- * links or loads this file.
+ * baseline).  Test and CPU-baseline infrastructure only: nothing in the
+ * product links or loads this file.
  *
  * qggmrf_add_f32 restates cpuTotalVar of test/tvd.cpp:39-75 (weights,
  * d_potfun, replicate-at-edge clamp), parallelised over slices and rows with

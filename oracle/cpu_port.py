@@ -1,8 +1,8 @@
 """FP32 CPU restatement of ``tomocam::mbir2`` used as the benchmark's CPU
 baseline.
 
-BASELINE INFRASTRUCTURE ONLY (imported by ``bench.py``'s cpu_baseline leg and
-by tests); the product never imports it.  The reference has no CPU path (it
+CPU-baseline infrastructure only (imported by ``bench.py``'s cpu_baseline
+leg and by tests); the product never imports it.  The reference has no CPU path (it
 links only cufinufft and cuFFT), so this is a project-written restatement of
 its algorithm, built to be fast on the host rather than a strawman:
 

@@ -1,7 +1,7 @@
 """Full mbir2 pipeline (normalise, FBP guess, pad, P^T y, PSF, step, run2,
 crop) on the GPU vs the FP64 oracle, plus its error behaviour.
 
-Tolerance: relative L2 <= 1e-4 on the reconstruction;
+Tolerance: relative L2 <= 1e-4 on the reconstruction (the parity bound);
 set-up scalars (Lipschitz estimate, ||y||^2) to 1e-5 relative.
 """
 
@@ -50,7 +50,7 @@ def test_mbir2_small_vs_oracle(torch_cuda, center_shift):
     assert rel_l2(out.cpu().numpy(), ref) < 1e-4
 
 
-def test_mbir2_config0_shape(torch_cuda):
+def test_mbir2_cpu_path_shape(torch_cuda):
     """The CPU-path workload: 4 x 256^2 Shepp-Logan, 180 angles, centre 128,
     1% noise, 50 iterations.  Compared at sigma = 10 (converging) against
     the FP64 oracle with the single-Toeplitz run2 restatement."""
@@ -64,7 +64,7 @@ def test_mbir2_config0_shape(torch_cuda):
     reg = lambda v: O.qggmrf_grad(v, sigma, 1.2)
     rec = O.run2_linear(st.x0, T, st.sinoT, reg, st.sino_norm, iters, st.step)
     ref = O.postproc(rec, N)
-    announce("mbir2 config0 shape")
+    announce("mbir2 cpu-path shape")
     out, info = gpu.mbir(torch.tensor(sino, dtype=torch.float32).cuda(), ang,
                          N / 2, iters, sigma)
     assert info["padded"] == 362

@@ -130,7 +130,7 @@ def test_mbir2_mixed_power_lipschitz(torch_cuda):
                                          rel=1e-12)
 
 
-def test_mbir2_mixed_vs_fp32_config1_shape(torch_cuda):
+def test_mbir2_mixed_vs_fp32_1024_shape(torch_cuda):
     """4 slices of the 32 x 1024^2 / 720-angle workload (padded 1454, FFT
     length 2916), 30 iterations, sigma = 100: mixed against FP32."""
     torch = torch_cuda

@@ -39,7 +39,7 @@ def test_project_backproject_exact(torch_cuda, N, P, S):
     assert rel_l2(got, O.backproject(sino, a64, fbp=True, exact=True)) < 1e-5
 
 
-def test_project_config0_geometry(torch_cuda):
+def test_project_cpu_path_geometry(torch_cuda):
     """256^2 Shepp-Logan, 180 angles (the CPU-path workload's geometry)
     against the FP64 ES-kernel oracle (w = 14)."""
     torch = torch_cuda

@@ -2,7 +2,7 @@
 
 The set-up (x0, sinoT, PSF, ||y||^2, step) comes from the oracle so this
 pins the iteration alone.  Tolerance: relative L2 <= 1e-4 on the solution
-and 1e-5 on the per-iteration error history.
+(the parity bound) and 1e-5 on the per-iteration error history.
 """
 
 import math

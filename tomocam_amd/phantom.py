@@ -69,15 +69,15 @@ def random_ellipses(n: int, seed: int, count: int = 30) -> np.ndarray:
 
 
 def phantom_stack(n: int, nslices: int, kind: str = "ellipses",
-                  config_index: int = 1) -> np.ndarray:
+                  seed_group: int = 1) -> np.ndarray:
     """[nslices, n, n] float32 stack; ellipse slices use seed
-    1000*config_index + slice."""
+    1000*seed_group + slice."""
     if kind == "shepp":
         sl = shepp_logan(n)
         return np.repeat(sl[None], nslices, axis=0).astype(np.float32)
     out = np.empty((nslices, n, n), dtype=np.float32)
     for s in range(nslices):
-        out[s] = random_ellipses(n, 1000 * config_index + s)
+        out[s] = random_ellipses(n, 1000 * seed_group + s)
     return out
 
 
