@@ -28,7 +28,7 @@ from tomocam_amd import phantom as PH  # noqa: E402
 
 ITERS, SIGMA = 30, 100.0
 # N: (projections, phantom seed, noise fraction, noise seed)
-CONFIGS = {1024: (720, 1000, 0.01, 1007), 2048: (1501, 3000, 0.005, 3007)}
+SHAPES = {1024: (720, 1000, 0.01, 1007), 2048: (1501, 3000, 0.005, 3007)}
 
 
 def fixture_path(N):
@@ -36,7 +36,7 @@ def fixture_path(N):
 
 
 def make_sinogram(N):
-    P, seed, frac, nseed = CONFIGS[N]
+    P, seed, frac, nseed = SHAPES[N]
     ang = PH.angles(P).astype(np.float64)
     img = PH.random_ellipses(N, seed)[None]
     sino = O.project(img, ang)

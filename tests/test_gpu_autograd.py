@@ -11,7 +11,7 @@ FP64 oracle's P^T (P x - y).
   along 64 random probe directions match FP64 central finite differences
   (eps = 1e-6) of the oracle loss: each within 1e-4 |grad| |d| and relL2
   over the probes <= 1e-4.
-- Configuration-2 shape (8 x 512^2, 360 angles, beta = 1e-3): the gradient
+- The differentiation workload's shape (8 x 512^2, 360 angles, beta = 1e-3): the gradient
   matches the oracle on the first two slices (relL2 <= 1e-4; the prior
   term uses the whole volume).
 """
@@ -108,7 +108,8 @@ def test_float64_input(torch_cuda):
 
 
 def test_config2_shape(torch_cuda):
-    """8 slices of 512^2, 360 angles, beta = 1e-3 (configuration 2)."""
+    """8 slices of 512^2, 360 angles, beta = 1e-3 (the differentiation
+    workload's shape)."""
     S, N, P, beta, sigma = 8, 512, 360, 1e-3, 0.1
     ang = PH.angles(P).astype(np.float64)
     x = np.stack([PH.random_ellipses(N, 2000 + s) for s in range(S)])
@@ -116,7 +117,7 @@ def test_config2_shape(torch_cuda):
     from tomocam_amd import gpu
     y = gpu.project(torch_cuda.tensor(xs, dtype=torch_cuda.float32,
                                       device="cuda"), ang).cpu().numpy()
-    announce("autograd configuration 2 (8 x 512^2, 360 angles)")
+    announce("autograd 8 x 512^2, 360 angles")
     _, g = _loss_grad(torch_cuda, x, y, ang, beta, sigma, torch_cuda.float32)
     x32 = x.astype(np.float32).astype(np.float64)
     data = O.backproject(O.project(x32[:2], ang) - y[:2], ang)
