@@ -413,18 +413,24 @@ ct_rows_inv(const void* __restrict__ wtv, const float* bvec, float* out,
   const bool hasa = ra < n, hasb = ra + 1 < n;
   // FP16 input: the transform is in units of gsc times the true values
   const float gi = H16 ? 1.0f / ha.gsc[s] : 1.0f;
-  // epilogue operands (b, or T sol and T x) all loaded before any store,
-  // which the compiler could not otherwise move past (out may alias Tsol).
-  // (Issuing them before the transform measured slower: more registers.)
-  float2 e0[MODE != 0 ? E : 1], e1[MODE == 3 ? E : 1];
-  if constexpr (MODE != 0) {
-    const float* src0 = MODE == 1 ? bvec : cp.Tsol;
+  // Epilogue in NB batches: each batch's operands (b, or T sol and T x) are
+  // all loaded before its stores, which the compiler could not otherwise
+  // move past (out may alias Tsol).  Two batches instead of one take the
+  // 2916-point MODE 2 kernel from 138 to 107 VGPRs, so two workgroups fit
+  // on a CU: 16 % faster at 32 x 1454^2.  (Issuing the loads before the
+  // transform measured slower: more registers.)
+  constexpr int NB = 2, EB = E / NB;
+  static_assert(E % NB == 0, "epilogue batches must divide E");
+  const float se = (MODE >= 2 ? cp.s : 1.0f) * gi;
 #pragma unroll
-    for (int i = 0; i < E / RS; ++i)
+  for (int h = 0; h < NB; ++h) {
+    float2 e0[MODE != 0 ? EB : 1], e1[MODE == 3 ? EB : 1];
+    if constexpr (MODE != 0) {
+      const float* src0 = MODE == 1 ? bvec : cp.Tsol;
 #pragma unroll
-      for (int r = 0; r < RS; ++r) {
+      for (int j = 0; j < EB; ++j) {
+        const int e = h * EB + j, i = e / RS, r = e - i * RS;
         const int c = t + i * T + r * (L / RS);
-        const int e = i * RS + r;
         float2 p0 = make_float2(0.f, 0.f), p1 = p0;
         if (c < n) {
           if (hasa) p0.x = src0[oa + c];
@@ -434,17 +440,14 @@ ct_rows_inv(const void* __restrict__ wtv, const float* bvec, float* out,
             if (hasb) p1.y = cp.Tx[oa + n + c];
           }
         }
-        e0[e] = p0;
-        if constexpr (MODE == 3) e1[e] = p1;
+        e0[j] = p0;
+        if constexpr (MODE == 3) e1[j] = p1;
       }
-  }
-  const float se = (MODE >= 2 ? cp.s : 1.0f) * gi;
+    }
 #pragma unroll
-  for (int i = 0; i < E / RS; ++i)
-#pragma unroll
-    for (int r = 0; r < RS; ++r) {
+    for (int j = 0; j < EB; ++j) {
+      const int e = h * EB + j, i = e / RS, r = e - i * RS;
       const int c = t + i * T + r * (L / RS);
-      const int e = i * RS + r;
       if (c < n) {
         const float2 q = v[e];
         float2 o = q;
@@ -454,13 +457,13 @@ ct_rows_inv(const void* __restrict__ wtv, const float* bvec, float* out,
             o.y = q.y * gi;
           }
         } else if constexpr (MODE == 1) {
-          o.x = fmaf(q.x, gi, -e0[e].x);
-          o.y = fmaf(q.y, gi, -e0[e].y);
+          o.x = fmaf(q.x, gi, -e0[j].x);
+          o.y = fmaf(q.y, gi, -e0[j].y);
         } else {
-          float2 ty = e0[e];
+          float2 ty = e0[j];
           if constexpr (MODE == 3) {
-            ty.x = fmaf(cp.beta, ty.x - e1[e].x, ty.x);
-            ty.y = fmaf(cp.beta, ty.y - e1[e].y, ty.y);
+            ty.x = fmaf(cp.beta, ty.x - e1[j].x, ty.x);
+            ty.y = fmaf(cp.beta, ty.y - e1[j].y, ty.y);
           }
           o.x = fmaf(-se, q.x, ty.x);
           o.y = fmaf(-se, q.y, ty.y);
@@ -469,6 +472,7 @@ ct_rows_inv(const void* __restrict__ wtv, const float* bvec, float* out,
         if (hasb) out[oa + n + c] = o.y;
       }
     }
+  }
 }
 
 // ---------------------------------------------------------------------------
