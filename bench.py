@@ -64,7 +64,7 @@ WORKLOADS = {
         slices=64, npix=2048, nproj=1501, iterations=100, sigma=100.0, p=1.2,
         noise=0.005, seed_group=3, precision="mixed",
         label="mixed precision (FP16 gradient and Toeplitz intermediates, "
-              "FP32 FFTs and state, FP64 dots, FP32 refresh every 10): 64 "
+              "FP32 FFTs and state, FP64 dots, FP32 refresh every 5): 64 "
               "slices 2048x2048, 1501 angles x 2048 rays, 100 iterations, "
               "qGGMRF sigma=100 p=1.2"),
 }

@@ -379,7 +379,7 @@ DArray<T> mbir2(std::optional<DArray<T>> guess, const DArray<T>& sino,
 template <typename T>
 DArray<T> mbir2_mixed(std::optional<DArray<T>> guess, const DArray<T>& sino,
                       std::vector<T> angles, T center, int num_iters, T sigma,
-                      int refresh_every = 10) {
+                      int refresh_every = 5) {
   detail::require_float<T>();
   const int S = sino.nslices(), P = sino.nrows(), N = sino.ncols();
   detail::Dev d_sino(sino.begin(), sino.size());

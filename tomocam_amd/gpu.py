@@ -346,9 +346,15 @@ def pad2d(img: torch.Tensor, padding: int) -> torch.Tensor:
     return out
 
 
+# FP32 refresh interval of the mixed-precision MBIR: every 10 iterations
+# leaves 1.0e-4 relative L2 against FP64 on one 2048^2 slice (30 iterations,
+# sigma = 100); every 5 halves the drift of the FP16 T g recurrence.
+MIXED_REFRESH = 5
+
+
 def mbir(sino: torch.Tensor, angles, center: float, num_iters: int = 100,
          sigma: float = 100.0, tol: float = 1e-4, xtol: float = 1e-4,
-         refresh_every: int = 10, verbose: bool = False,
+         refresh_every: Optional[int] = None, verbose: bool = False,
          guess: Optional[torch.Tensor] = None, precision: str = "fp32",
          lipschitz_iters: int = 0):
     """tomocam::mbir2 on a device-resident sinogram [S, P, N] -> [S, N, N].
@@ -359,7 +365,8 @@ def mbir(sino: torch.Tensor, angles, center: float, num_iters: int = 100,
 
     ``precision="mixed"`` runs the mixed-precision form (FP16 gradient and
     Toeplitz intermediates, FP32 state and FFT arithmetic, FP64 dots, T sol
-    refreshed in FP32 every ``refresh_every`` iterations); info then also
+    refreshed in FP32 every ``refresh_every`` iterations, default 5 there
+    and 10 in FP32: FP16 T g drifts faster); info then also
     has ``saturated`` (slice-attempts whose FP16 gradient clipped) and
     ``backtracks``.  ``lipschitz_iters > 0`` (mixed only) takes L from that
     many power-iteration steps on T instead of max(T 1)."""
@@ -367,6 +374,8 @@ def mbir(sino: torch.Tensor, angles, center: float, num_iters: int = 100,
         raise ValueError("precision must be 'fp32' or 'mixed'")
     if lipschitz_iters and precision != "mixed":
         raise ValueError("lipschitz_iters is a mixed-precision option")
+    if refresh_every is None:
+        refresh_every = MIXED_REFRESH if precision == "mixed" else 10
     sino = _as3d(sino)
     S, P, N = sino.shape
     ang = _angles(angles)
