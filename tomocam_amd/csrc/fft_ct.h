@@ -115,6 +115,13 @@ template <int R, int DIR>
 struct DftC<R, DIR, false> {
   static constexpr int R1 = first_radix<R>();
   static constexpr int R2 = R / R1;
+  // b * w_R^e (DIR sign), e a compile-time constant
+  __device__ __forceinline__ static float2 twiddle(float2 b, long e) {
+    if (e % R == 0) return b;
+    const float c = (float)cx_cos2pi(e, R);
+    const float s = (float)(DIR * cx_sin2pi(e, R));
+    return make_float2(b.x * c - b.y * s, b.x * s + b.y * c);
+  }
   __device__ __forceinline__ static void run(float2* a) {
     float2 t[R];
 #pragma unroll
@@ -124,18 +131,25 @@ struct DftC<R, DIR, false> {
       for (int n1 = 0; n1 < R1; ++n1) b[n1] = a[R2 * n1 + n2];
       DftC<R1, DIR>::run(b);
 #pragma unroll
-      for (int k1 = 0; k1 < R1; ++k1) {
-        const long e = (long)n2 * k1;
-        if (e % R == 0) {
-          t[k1 * R2 + n2] = b[k1];
-        } else {
-          const float c = (float)cx_cos2pi(e, R);
-          const float s = (float)(DIR * cx_sin2pi(e, R));
-          t[k1 * R2 + n2] = make_float2(b[k1].x * c - b[k1].y * s,
-                                        b[k1].x * s + b[k1].y * c);
-        }
-      }
+      for (int k1 = 0; k1 < R1; ++k1)
+        t[k1 * R2 + n2] = twiddle(b[k1], (long)n2 * k1);
     }
+    finish(t, a);
+  }
+  // The same DFT when a[R/2 .. R-1] are zero (zero-padded input): with a
+  // radix-2 first factor each radix-2 sub-DFT is (a[n2], a[n2]).
+  __device__ __forceinline__ static void run_zero_upper(float2* a) {
+    static_assert(R1 == 2, "zero-upper form needs a radix-2 first factor");
+    float2 t[R];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      t[n2] = a[n2];
+      t[R2 + n2] = twiddle(a[n2], n2);
+    }
+    finish(t, a);
+  }
+  // second factor: radix-R2 DFTs of the twiddled columns
+  __device__ __forceinline__ static void finish(const float2* t, float2* a) {
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) {
       float2 c[R2];
@@ -183,9 +197,11 @@ struct RadixList {
 
 // One stage's arithmetic on registers v[E] holding inputs v[i R + r] =
 // x[j + r L/R], j = t + i T; twiddle then DFT.
-template <int L, int T, int E, int R, int Ns, int DIR>
+// ZIN (first stage only): inputs r >= R/2 of every butterfly are zero.
+template <int L, int T, int E, int R, int Ns, int DIR, bool ZIN = false>
 __device__ __forceinline__ void stage_compute(float2* v, int t,
                                               const float2* __restrict__ tw) {
+  static_assert(!ZIN || Ns == 1, "zero inputs only in the first stage");
   constexpr int NB = E / R;
   constexpr int step = L / (Ns * R);
 #pragma unroll
@@ -198,7 +214,10 @@ __device__ __forceinline__ void stage_compute(float2* v, int t,
         v[i * R + r] = DIR < 0 ? cmul(v[i * R + r], w) : cmulc(v[i * R + r], w);
       }
     }
-    DftC<R, DIR>::run(v + i * R);
+    if constexpr (ZIN)
+      DftC<R, DIR>::run_zero_upper(v + i * R);
+    else
+      DftC<R, DIR>::run(v + i * R);
   }
 }
 
@@ -232,7 +251,7 @@ __device__ __forceinline__ void stage_read(float2* v, const float2* line,
 // registers; later stages first exchange through `line`.  Every thread of
 // the workgroup must call this (it contains barriers); `active` masks
 // threads beyond the lines of a partial workgroup.
-template <int L, int T, int E, int DIR, int s, int... Rs>
+template <int L, int T, int E, int DIR, bool ZIN, int s, int... Rs>
 struct StagesDispatch {
   using RL = RadixList<Rs...>;
   __device__ __forceinline__ static void run(float2* v, float2* line, int t,
@@ -249,20 +268,24 @@ struct StagesDispatch {
         if (active) stage_read<L, T, E, R>(v, line, t);
         __syncthreads();
       }
-      if (active) stage_compute<L, T, E, R, Ns, DIR>(v, t, tw);
-      StagesDispatch<L, T, E, DIR, s + 1, Rs...>::run(v, line, t, active, tw);
+      if (active) stage_compute<L, T, E, R, Ns, DIR, ZIN && s == 0>(v, t, tw);
+      StagesDispatch<L, T, E, DIR, ZIN, s + 1, Rs...>::run(v, line, t, active,
+                                                           tw);
     }
   }
 };
 
 // Full transform on registers.  Before: v[i R1 + r] = x[t + i T + r L/R1].
-// After: v[i RS + r] = X[t + i T + r L/RS].
-template <int L, int E, int DIR, int... Rs>
+// After: v[i RS + r] = X[t + i T + r L/RS].  ZIN: the input is zero from
+// index L/2 on (a zero-padded line), so v[i R1 + r] for r >= R1/2 are zero
+// and are not read; needs an even R1 whose first factor is 2.
+template <int L, int E, int DIR, bool ZIN, int... Rs>
 __device__ __forceinline__ void transform(float2* v, float2* line, int t,
                                           bool active,
                                           const float2* __restrict__ tw) {
   static_assert(plan_valid<L, E, Rs...>(), "invalid FFT plan");
-  StagesDispatch<L, L / E, E, DIR, 0, Rs...>::run(v, line, t, active, tw);
+  StagesDispatch<L, L / E, E, DIR, ZIN, 0, Rs...>::run(v, line, t, active,
+                                                       tw);
 }
 
 // index of register element (i, r) for a stage of radix R

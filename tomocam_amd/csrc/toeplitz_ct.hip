@@ -22,21 +22,32 @@ template <int L, int E, int DIR, int... Rs>
 struct Xform<L, E, DIR, Radices<Rs...>> {
   static constexpr int first = ct::RadixList<Rs...>::at(0);
   static constexpr int last = ct::RadixList<Rs...>::at(sizeof...(Rs) - 1);
+  // ZIN: input zero from index L/2 on (every line here is zero-padded
+  // from n <= L/2), so the first stage skips those terms
+  template <bool ZIN = false>
   __device__ __forceinline__ static void run(float2* v, float2* line, int t,
                                              bool active,
                                              const float2* __restrict__ tw) {
-    ct::transform<L, E, DIR, Rs...>(v, line, t, active, tw);
+    ct::transform<L, E, DIR, ZIN, Rs...>(v, line, t, active, tw);
   }
 };
 
+// Zero padding: a line holds n <= L/2 data values (L >= 2n - 1, L even).
+// Element r of a radix-R butterfly sits at index j + r L/R with
+// j < L/R, so for r >= R/2 it lies in the padding: stage-1 inputs there are
+// zero and last-stage outputs there are not kept.
+
 // Plans: forward radices Fwd, inverse radices Inv = reversed Fwd, LPW lines
-// per workgroup in the row passes and LPWC in the column pass.
-template <int L_, int E_, int LPW_, class Fwd_, class Inv_, int LPWC_ = LPW_>
+// per workgroup in the row passes and LPWC in the column pass.  ZA: pass A
+// skips the zero padding of its input in the first FFT stage.
+template <int L_, int E_, int LPW_, class Fwd_, class Inv_, int LPWC_ = LPW_,
+          bool ZA_ = true>
 struct TPlan {
   static constexpr int L = L_, E = E_, LPW = LPW_, T = L_ / E_;
   static constexpr int NT = LPW_ * (L_ / E_);
   static constexpr int LPWC = LPWC_;
   static constexpr int NTC = LPWC_ * (L_ / E_);
+  static constexpr bool ZA = ZA_;
   using Fwd = Fwd_;
   using Inv = Inv_;
 };
@@ -55,8 +66,10 @@ using Plan2250 = TPlan<2250, 30, 3, Radices<10, 15, 15>, Radices<15, 15, 10>>;
 #endif
 // Column pass: one column per workgroup measured 5 % faster than three at
 // this length; at 2250 three per workgroup stay 20 % faster than one.
+// Pass A without the zero-padding shortcut: with it, 32 x 1454^2 measured
+// 8 % slower in pass A (at 5832 the shortcut is 3 % faster).
 using Plan2916 = TPlan<2916, TC_P2916_E, TC_P2916_LPW, Radices<18, 18, 9>,
-                       Radices<9, 18, 18>, TC_P2916_LPWC>;
+                       Radices<9, 18, 18>, TC_P2916_LPWC, false>;
 using Plan4500 = TPlan<4500, 30, 2, Radices<10, 15, 30>, Radices<30, 15, 10>>;
 using Plan5832 = TPlan<5832, 18, 1, Radices<18, 18, 18>, Radices<18, 18, 18>>;
 
@@ -123,6 +136,7 @@ ct_rows_fwd(const void* __restrict__ xv, void* __restrict__ wtv,
     else
       return reinterpret_cast<const float*>(xv)[o];
   };
+  static_assert(R1 % 2 == 0, "plans need an even first radix");
 #pragma unroll
   for (int i = 0; i < E / R1; ++i)
 #pragma unroll
@@ -130,7 +144,7 @@ ct_rows_fwd(const void* __restrict__ xv, void* __restrict__ wtv,
       const int c = t + i * T + r * (L / R1);
       const int e = i * R1 + r;
       float re = 0.f, im = 0.f;
-      if (c < n) {
+      if ((!P::ZA || r < R1 / 2) && c < n) {
         if (hasa) re = ld(oa + c);
         if (hasb) im = ld(oa + n + c);
         if constexpr (CM != 0) {
@@ -155,7 +169,7 @@ ct_rows_fwd(const void* __restrict__ xv, void* __restrict__ wtv,
 #pragma unroll
     for (int i = 0; i < E / R1; ++i)
 #pragma unroll
-      for (int r = 0; r < R1; ++r) {
+      for (int r = 0; r < (P::ZA ? R1 / 2 : R1); ++r) {
         const int c = t + i * T + r * (L / R1);
         const int e = i * R1 + r;
         if (c < n) {
@@ -165,7 +179,7 @@ ct_rows_fwd(const void* __restrict__ xv, void* __restrict__ wtv,
       }
   }
   float2* ln = sm + line * L;
-  F::run(v, ln, t, true, tw);
+  F::template run<P::ZA>(v, ln, t, true, tw);
 #pragma unroll
   for (int i = 0; i < E / RS; ++i)
 #pragma unroll
@@ -234,7 +248,7 @@ ct_cols(void* __restrict__ wtv, const void* __restrict__ specv,
     for (int r = 0; r < R1; ++r) {
       const int idx = t + i * T + r * (L / R1);
       float2 w = make_float2(0.f, 0.f);
-      if (active && idx < n) {
+      if (r < R1 / 2 && active && idx < n) {
         const long o = wt_at(s, kc, idx, H, rp);
         if constexpr (H16)
           w = unpack_half2(reinterpret_cast<const unsigned*>(wtv)[o]);
@@ -260,7 +274,7 @@ ct_cols(void* __restrict__ wtv, const void* __restrict__ specv,
       }
   }
   float2* ln = sm + line * L;
-  F::run(v, ln, t, active, tw);
+  F::template run<true>(v, ln, t, active, tw);
   double acc = 0.0;
   if (active) {
     const float wgt = (k1 == 0 || 2 * k1 == L) ? 1.f : 2.f;
@@ -286,7 +300,7 @@ ct_cols(void* __restrict__ wtv, const void* __restrict__ specv,
 #pragma unroll
       for (int i = 0; i < E / R1; ++i)
 #pragma unroll
-        for (int r = 0; r < R1; ++r) {
+        for (int r = 0; r < R1 / 2; ++r) {
           const int idx = t + i * T + r * (L / R1);
           if (idx < n)
             m = fmaxf(m, fmaxf(fabsf(v[i * R1 + r].x), fabsf(v[i * R1 + r].y)));
@@ -302,7 +316,7 @@ ct_cols(void* __restrict__ wtv, const void* __restrict__ specv,
 #pragma unroll
       for (int i = 0; i < E / R1; ++i)
 #pragma unroll
-        for (int r = 0; r < R1; ++r) {
+        for (int r = 0; r < R1 / 2; ++r) {
           const int idx = t + i * T + r * (L / R1);
           if (idx < n)
             reinterpret_cast<unsigned*>(wtv)[wt_at(s, kc, idx, H, rp)] =
@@ -314,7 +328,7 @@ ct_cols(void* __restrict__ wtv, const void* __restrict__ specv,
 #pragma unroll
     for (int i = 0; i < E / R1; ++i)
 #pragma unroll
-      for (int r = 0; r < R1; ++r) {
+      for (int r = 0; r < R1 / 2; ++r) {
         const int idx = t + i * T + r * (L / R1);
         if (idx < n)
           reinterpret_cast<float2*>(wtv)[wt_at(s, kc, idx, H, rp)] =
@@ -421,6 +435,7 @@ ct_rows_inv(const void* __restrict__ wtv, const float* bvec, float* out,
   // transform measured slower: more registers.)
   constexpr int NB = 2, EB = E / NB;
   static_assert(E % NB == 0, "epilogue batches must divide E");
+  static_assert(RS % 2 == 0, "plans need an even last inverse radix");
   const float se = (MODE >= 2 ? cp.s : 1.0f) * gi;
 #pragma unroll
   for (int h = 0; h < NB; ++h) {
@@ -432,7 +447,7 @@ ct_rows_inv(const void* __restrict__ wtv, const float* bvec, float* out,
         const int e = h * EB + j, i = e / RS, r = e - i * RS;
         const int c = t + i * T + r * (L / RS);
         float2 p0 = make_float2(0.f, 0.f), p1 = p0;
-        if (c < n) {
+        if (r < RS / 2 && c < n) {
           if (hasa) p0.x = src0[oa + c];
           if (hasb) p0.y = src0[oa + n + c];
           if constexpr (MODE == 3) {
@@ -448,7 +463,7 @@ ct_rows_inv(const void* __restrict__ wtv, const float* bvec, float* out,
     for (int j = 0; j < EB; ++j) {
       const int e = h * EB + j, i = e / RS, r = e - i * RS;
       const int c = t + i * T + r * (L / RS);
-      if (c < n) {
+      if (r < RS / 2 && c < n) {
         const float2 q = v[e];
         float2 o = q;
         if constexpr (MODE == 0) {
