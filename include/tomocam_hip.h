@@ -316,6 +316,15 @@ int tc_ktimer_enable(int kernel_class, int on);
 int tc_ktimer_collect(int kernel_class, double* total_ms, long* count,
                       double* bytes);
 
+/* Number of MBIR attempts this process has replayed from HIP graphs (no
+ * counterpart in the reference).  With TOMOCAM_HIP_GRAPHS=1 the MBIR entry
+ * points capture the steady-state attempt (y = sol, full step) once per
+ * buffer role and replay it, except with kernel timing enabled, an FFT
+ * length without compile-time passes, the separate commit kernel, a second
+ * Toeplitz stream, or concurrent tc_mbir2_batch workers.  Off by default:
+ * it measured no faster than launching the kernels. */
+int tc_graph_replays(long* count);
+
 #ifdef __cplusplus
 }
 #endif

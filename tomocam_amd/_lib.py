@@ -32,6 +32,7 @@ SIGNATURES = {
     "tc_psf_destroy": [vp],
     "tc_psf_info": [vp, c_int_p, c_int_p],
     "tc_toeplitz_fft_length": [C.c_int, c_int_p],
+    "tc_graph_replays": [C.POINTER(C.c_long)],
     "tc_toeplitz_apply": [vp, vp, vp, C.c_int, c_double_p, vp],
     "tc_toeplitz_apply_mixed": [vp, vp, vp, C.c_int, c_double_p, vp],
     "tc_psf_tune": [vp, C.c_int, C.c_int, c_int_p, vp],

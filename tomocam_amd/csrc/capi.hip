@@ -477,6 +477,9 @@ int tc_mbir2_batch(int nstacks, const float* const* d_sinos,
     // timings taken while other stacks run would be noise: concurrent
     // stacks use tuned sizes from earlier calls, or the default rule
     o.autotune = false;
+    // a worker capturing a graph would make the others' synchronous copies
+    // (plan set-up) fail
+    o.allow_graphs = false;
     int dev = 0;
     TC_HIP(hipGetDevice(&dev));
     std::vector<int> order(nstacks);
@@ -573,6 +576,13 @@ int tc_mbir2_host(const float* h_sino, float* h_recon, int nslices, int nproj,
 
 int tc_ktimer_enable(int kernel_class, int on) {
   return guarded([&] { tc::ktimer_enable(kernel_class, on != 0); });
+}
+
+int tc_graph_replays(long* count) {
+  return guarded([&] {
+    TC_REQUIRE(count, "graph replays: null output");
+    *count = tc::graph_replays();
+  });
 }
 
 int tc_ktimer_collect(int kernel_class, double* total_ms, long* count,

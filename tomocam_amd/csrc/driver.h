@@ -41,6 +41,8 @@ struct Mbir2Options {
   // time the Toeplitz slice-group sizes once per shape (Psf::tune_groups);
   // TOMOCAM_HIP_AUTOTUNE=0 turns it off process-wide
   bool autotune = true;
+  // HIP-graph replay of the steady-state attempt (Run2Options::graphs)
+  bool allow_graphs = true;
 };
 
 struct Mbir2Result {

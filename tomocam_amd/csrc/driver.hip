@@ -209,6 +209,7 @@ Mbir2Result mbir2(const float* sino, const float* x0, float* recon, int S,
   ro.refresh_every = o.refresh_every;
   ro.verbose = o.verbose;
   ro.mixed = o.mixed;
+  ro.allow_graphs = o.allow_graphs;
   DevBuf bsol(sizeof(float) * (size_t)S * n2 * n2);
   res.stats = run2_device(psf, bx0p.as<float>(), bb.as<float>(),
                           bsol.as<float>(), S, ro, stream);

@@ -47,6 +47,11 @@ struct Run2Options {
   // refresh above recomputes T sol in FP32, bounding the drift that the
   // FP16 T g leaves in the T y recurrence.
   bool mixed = false;
+  // Replay the steady-state attempt from HIP graphs when the environment
+  // sets TOMOCAM_HIP_GRAPHS=1 (off by default: it measured slower, see
+  // run2_device).  false keeps it off regardless, for callers whose other
+  // threads make synchronous HIP calls while this one might be capturing.
+  bool allow_graphs = true;
   int verbose = 0;         // print the reference's per-iteration line
   // optional per-iteration record: (iter, error, x-err, step, backtracks)
   std::function<void(int, double, double, double, int)> on_iter;
@@ -67,5 +72,8 @@ struct Run2Stats {
 // out: solution [S][n][n] (device, written).
 Run2Stats run2_device(Psf& psf, const float* x0, const float* b, float* out,
                       int S, const Run2Options& opt, hipStream_t stream);
+
+// Attempts replayed from HIP graphs by run2_device in this process.
+long graph_replays();
 
 }  // namespace tc

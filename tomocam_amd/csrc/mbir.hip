@@ -1,10 +1,14 @@
 // qGGMRF stencil, fused MBIR gradient/commit kernels and the run2 driver.
 #include <hip/hip_fp16.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <stdexcept>
+#include <string>
 #include <utility>
+#include <vector>
 
 #include "lds_launch.h"
 #include "mbir.h"
@@ -652,8 +656,93 @@ void to_half_slices(const float* x, void* x16, float* gsc, int S, long plane,
 // candidate on every attempt (accepted or not), `x` takes it only on
 // acceptance, so after an acceptance x == sol and the next first attempt has
 // y = sol.
+// The stream the MBIR loop runs on.  HIP graphs need a stream that can be
+// captured (not the null stream), so with graphs on the loop runs on a
+// private non-blocking stream (default priority) ordered after the caller's
+// work; the destructor drains it before the buffers declared earlier are
+// freed.
+struct LoopStream {
+  hipStream_t s = nullptr;
+  bool own = false;
+  LoopStream(hipStream_t caller, bool private_stream) : s(caller) {
+    if (!private_stream) return;
+    hipEvent_t ev;
+    TC_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) {
+      own = true;
+      e = hipEventRecord(ev, caller);
+      if (e == hipSuccess) e = hipStreamWaitEvent(s, ev, 0);
+    }
+    (void)hipEventDestroy(ev);
+    if (e != hipSuccess) {
+      if (own) (void)hipStreamDestroy(s);
+      throw std::runtime_error(std::string("HIP error: ") +
+                               hipGetErrorString(e));
+    }
+  }
+  ~LoopStream() {
+    if (!own) return;
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+  }
+  LoopStream(const LoopStream&) = delete;
+  LoopStream& operator=(const LoopStream&) = delete;
+};
+
+// Captures the work `enqueue` puts on `stream` into an executable graph;
+// nullptr (and no error left pending) if any step of the capture fails.
+template <class F>
+static hipGraphExec_t capture_attempt(hipStream_t stream, F&& enqueue) {
+  if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) !=
+      hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  hipGraph_t graph = nullptr;
+  bool ok = true;
+  try {
+    (void)enqueue();
+  } catch (...) {
+    ok = false;
+  }
+  if (hipStreamEndCapture(stream, &graph) != hipSuccess) ok = false;
+  hipGraphExec_t exec = nullptr;
+  if (ok && graph &&
+      hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+    exec = nullptr;
+  if (graph) (void)hipGraphDestroy(graph);
+  (void)hipGetLastError();
+  return exec;
+}
+
+static std::atomic<long> g_graph_replays{0};
+long graph_replays() { return g_graph_replays.load(); }
+
+// Instantiated graphs of one steady-state attempt, keyed by the buffer
+// roles they were captured with.
+struct AttemptGraphs {
+  struct Entry {
+    const float* sol;
+    const float* Tsol;
+    const float* gsc;
+    bool fused;
+    hipGraphExec_t exec;
+  };
+  std::vector<Entry> e;
+  Entry* find(const float* sol, const float* Tsol, const float* gsc) {
+    for (auto& x : e)
+      if (x.sol == sol && x.Tsol == Tsol && x.gsc == gsc) return &x;
+    return nullptr;
+  }
+  ~AttemptGraphs() {
+    for (auto& x : e)
+      if (x.exec) (void)hipGraphExecDestroy(x.exec);
+  }
+};
+
 Run2Stats run2_device(Psf& psf, const float* x0, const float* b, float* out,
-                      int S, const Run2Options& opt, hipStream_t stream) {
+                      int S, const Run2Options& opt, hipStream_t caller) {
   const int n = psf.n();
   TC_REQUIRE(S >= 1, "run2: need at least one slice");
   TC_REQUIRE(opt.num_iters >= 0, "run2: num_iters must be >= 0");
@@ -684,6 +773,32 @@ Run2Stats run2_device(Psf& psf, const float* x0, const float* b, float* out,
   // mixed precision: FP16 gradient, per-slice scales (this attempt's and
   // the next one's), per-wave maxima, saturation counter
   DevBuf bg16, bgsc, bgmax, bsat;
+  DevBuf dots_dev(sizeof(double) * 8);
+  double* dots_h = nullptr;
+  TC_HIP(hipHostMalloc((void**)&dots_h, sizeof(double) * 8, hipHostMallocDefault));
+  struct HostFree {
+    double* p;
+    ~HostFree() { if (p) (void)hipHostFree(p); }
+  } hf{dots_h};
+  // TOMOCAM_HIP_FUSED_COMMIT=0 selects the separate commit kernel (same
+  // arithmetic, one more pass over HBM)
+  const char* fenv = std::getenv("TOMOCAM_HIP_FUSED_COMMIT");
+  const bool allow_fused = !(fenv && fenv[0] == '0');
+  // With TOMOCAM_HIP_GRAPHS=1, steady-state attempts (y = sol, step s0)
+  // replay a HIP graph per buffer role.  Not with kernel timing (its events
+  // are per launch), the separate commit kernel or a second Toeplitz
+  // stream.  Off by default: per call it adds a stream and up to eight
+  // captures and instantiations, and per attempt a graph launch measured
+  // no faster than the 6-15 launches it replaces (4 x 362^2, 50
+  // iterations: 7.8 ms against 7.2 ms eager; 32 x 1454^2: equal).
+  const char* genv = std::getenv("TOMOCAM_HIP_GRAPHS");
+  bool use_graphs = opt.allow_graphs && genv && genv[0] == '1' &&
+                    !ktimer_active() && psf.has_fused() &&
+                    (allow_fused || opt.mixed) &&
+                    psf.single_stream(S, opt.mixed);
+  AttemptGraphs graphs;
+  LoopStream loop_stream(caller, use_graphs);
+  const hipStream_t stream = loop_stream.s;
   float *gsc = nullptr, *gsc_next = nullptr;
   if (opt.mixed) {
     TC_REQUIRE(psf.has_fused(),
@@ -703,23 +818,11 @@ Run2Stats run2_device(Psf& psf, const float* x0, const float* b, float* out,
     TC_HIP(hipGetLastError());
   }
   const int ntiles_q = (int)(qblk / cdiv(S, qa.zchunk));
-  DevBuf dots_dev(sizeof(double) * 8);
-  double* dots_h = nullptr;
-  TC_HIP(hipHostMalloc((void**)&dots_h, sizeof(double) * 8, hipHostMallocDefault));
-  struct HostFree {
-    double* p;
-    ~HostFree() { if (p) (void)hipHostFree(p); }
-  } hf{dots_h};
 
   // sol = x = x0, T sol = T x = T x0 (aliased until the first rejection)
   TC_HIP(hipMemcpyAsync(A, x0, vb, hipMemcpyDeviceToDevice, stream));
   psf.apply(A, nullptr, TA, S, 0, nullptr, stream);
   float *sol = A, *xp = A, *Tsol = TA, *Tx = TA;
-
-  // TOMOCAM_HIP_FUSED_COMMIT=0 selects the separate commit kernel (same
-  // arithmetic, one more pass over HBM)
-  const char* fenv = std::getenv("TOMOCAM_HIP_FUSED_COMMIT");
-  const bool allow_fused = !(fenv && fenv[0] == '0');
 
   Run2Stats st;
   st.toeplitz_applies = 1;
@@ -742,31 +845,6 @@ Run2Stats run2_device(Psf& psf, const float* x0, const float* b, float* out,
       qa.Tx = has_x ? Tx : Tsol;
       qa.beta = has_x ? (float)beta : 0.0f;
       qa.gsc = gsc;
-      ktimer_begin(KC_MBIR_GRADIENT, stream);
-      if (opt.mixed) {
-        if (has_x)
-          hipLaunchKernelGGL((qggmrf_kernel<1, true, true>), dim3(qblk),
-                             dim3(256), 0, stream, qa);
-        else
-          hipLaunchKernelGGL((qggmrf_kernel<1, false, true>), dim3(qblk),
-                             dim3(256), 0, stream, qa);
-      } else if (has_x) {
-        hipLaunchKernelGGL((qggmrf_kernel<1, true>), dim3(qblk), dim3(256), 0,
-                           stream, qa);
-      } else {
-        hipLaunchKernelGGL((qggmrf_kernel<1, false>), dim3(qblk), dim3(256), 0,
-                           stream, qa);
-      }
-      TC_HIP(hipGetLastError());
-      // reads sol, T sol, b (and x, T x unless aliased), writes g
-      ktimer_end(KC_MBIR_GRADIENT, stream,
-                 (has_x ? 5.0 : 3.0) * vb + (opt.mixed ? 0.5 : 1.0) * vb);
-      if (opt.mixed) {
-        hipLaunchKernelGGL(gscale_update_kernel, dim3(S), dim3(256), 0, stream,
-                           qa.gmaxp, ntiles_q, qa.zchunk, gsc, gsc_next,
-                           bsat.as<int>());
-        TC_HIP(hipGetLastError());
-      }
       // The step is committed inside the Toeplitz passes where the FFT size
       // allows: cand = y - s g and T cand = T y - s T g are written before
       // the accept test, which then only decides the buffers' roles.  In
@@ -776,19 +854,71 @@ Run2Stats run2_device(Psf& psf, const float* x0, const float* b, float* out,
       const Psf::StepCommit sc{sol, has_x ? xp : nullptr, Tsol,
                                has_x ? Tx : nullptr, fdst, fTdst,
                                has_x ? (float)beta : 0.0f, (float)step};
-      const bool fused =
-          opt.mixed
-              ? psf.apply_commit_half(qa.g16, gsc, sc, S, tparts, stream)
-              : allow_fused && psf.apply_commit(g, sc, S, tparts, stream);
+      // one attempt's work: stencil (+ FP16 scales), Toeplitz passes with
+      // the commit, dot reduction, dots to the host; returns whether the
+      // commit was fused
+      auto enqueue = [&]() -> bool {
+        ktimer_begin(KC_MBIR_GRADIENT, stream);
+        if (opt.mixed) {
+          if (has_x)
+            hipLaunchKernelGGL((qggmrf_kernel<1, true, true>), dim3(qblk),
+                               dim3(256), 0, stream, qa);
+          else
+            hipLaunchKernelGGL((qggmrf_kernel<1, false, true>), dim3(qblk),
+                               dim3(256), 0, stream, qa);
+        } else if (has_x) {
+          hipLaunchKernelGGL((qggmrf_kernel<1, true>), dim3(qblk), dim3(256),
+                             0, stream, qa);
+        } else {
+          hipLaunchKernelGGL((qggmrf_kernel<1, false>), dim3(qblk), dim3(256),
+                             0, stream, qa);
+        }
+        TC_HIP(hipGetLastError());
+        // reads sol, T sol, b (and x, T x unless aliased), writes g
+        ktimer_end(KC_MBIR_GRADIENT, stream,
+                   (has_x ? 5.0 : 3.0) * vb + (opt.mixed ? 0.5 : 1.0) * vb);
+        if (opt.mixed) {
+          hipLaunchKernelGGL(gscale_update_kernel, dim3(S), dim3(256), 0,
+                             stream, qa.gmaxp, ntiles_q, qa.zchunk, gsc,
+                             gsc_next, bsat.as<int>());
+          TC_HIP(hipGetLastError());
+        }
+        const bool f =
+            opt.mixed
+                ? psf.apply_commit_half(qa.g16, gsc, sc, S, tparts, stream)
+                : allow_fused && psf.apply_commit(g, sc, S, tparts, stream);
+        if (!f) psf.apply(g, nullptr, Tg, S, 0, tparts, stream);
+        hipLaunchKernelGGL(reduce_dots_kernel, dim3(NDOT + 1), dim3(1024), 0,
+                           stream, qparts, (long)qblk, NDOT, tparts,
+                           (long)ntp, dots_dev.as<double>());
+        TC_HIP(hipGetLastError());
+        TC_HIP(hipMemcpyAsync(dots_h, dots_dev.as<double>(),
+                              sizeof(double) * 8, hipMemcpyDeviceToHost,
+                              stream));
+        return f;
+      };
+      // Steady state (y = sol, step s0): the first attempt with given buffer
+      // roles runs eagerly (first-use allocations happen there), the second
+      // captures a graph, later ones replay it.  A capture that fails turns
+      // graphs off for the rest of the call.
+      bool fused;
+      AttemptGraphs::Entry* ge =
+          use_graphs && !has_x ? graphs.find(sol, Tsol, gsc) : nullptr;
+      if (ge && !ge->exec) {
+        ge->exec = capture_attempt(stream, enqueue);
+        if (!ge->exec) use_graphs = false;
+      }
+      if (ge && ge->exec) {
+        TC_HIP(hipGraphLaunch(ge->exec, stream));
+        g_graph_replays.fetch_add(1);
+        fused = ge->fused;
+      } else {
+        fused = enqueue();
+        if (use_graphs && !has_x && !ge)
+          graphs.e.push_back({sol, Tsol, gsc, fused, nullptr});
+      }
       if (opt.mixed) std::swap(gsc, gsc_next);
-      if (!fused) psf.apply(g, nullptr, Tg, S, 0, tparts, stream);
       ++st.toeplitz_applies;
-      hipLaunchKernelGGL(reduce_dots_kernel, dim3(NDOT + 1), dim3(1024), 0,
-                         stream, qparts, (long)qblk, NDOT, tparts, (long)ntp,
-                         dots_dev.as<double>());
-      TC_HIP(hipGetLastError());
-      TC_HIP(hipMemcpyAsync(dots_h, dots_dev.as<double>(), sizeof(double) * 8,
-                            hipMemcpyDeviceToHost, stream));
       TC_HIP(hipStreamSynchronize(stream));
       ++st.attempts;
       const double yTy = dots_h[0], yb = dots_h[1], gR = dots_h[2];

@@ -34,6 +34,14 @@ void ktimer_enable(int kc, bool on) {
   kt().enabled[kc] = on;
 }
 
+bool ktimer_active() {
+  KTimer& k = kt();
+  std::lock_guard<std::mutex> lk(k.mu);
+  for (bool on : k.enabled)
+    if (on) return true;
+  return false;
+}
+
 void ktimer_begin(int kc, hipStream_t stream) {
   KTimer& k = kt();
   if (!k.enabled[kc]) return;

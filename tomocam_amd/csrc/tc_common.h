@@ -128,6 +128,8 @@ enum KernelClass : int {
 // outputs written once), accumulated for the roofline report.
 void ktimer_begin(int kc, hipStream_t stream);
 void ktimer_end(int kc, hipStream_t stream, double bytes = 0.0);
+// True if timing is enabled for any kernel class.
+bool ktimer_active();
 
 // ---------------------------------------------------------------------------
 // device complex helpers (float2 = re, im)

@@ -96,6 +96,10 @@ class Psf {
   // Grows the workspace for `nslices` slices.
   void reserve(int nslices);
 
+  // True when the passes for `nslices` slices (FP16 storage if `half`) run
+  // on the caller's stream alone, with no second stream to order.
+  bool single_stream(int nslices, bool half) const;
+
  private:
   // the two-stream set-up when TOMOCAM_HIP_TOEPLITZ_STREAMS (or the tuner)
   // asks for two streams for this slice count, else nullptr

@@ -275,6 +275,10 @@ Psf::~Psf() {
   if (ts_.done) (void)hipEventDestroy(ts_.done);
 }
 
+bool Psf::single_stream(int nslices, bool half) const {
+  return toeplitz_streams(H_, rp_, half ? 4 : 8, nslices) < 2;
+}
+
 const ToeplitzStreams* Psf::streams_for(int nslices, int elem_bytes) {
   if (toeplitz_streams(H_, rp_, elem_bytes, nslices) < 2) return nullptr;
   if (!ts_.aux) {

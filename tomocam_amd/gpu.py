@@ -218,6 +218,13 @@ def run2(psf: PointSpreadFunction, x0: torch.Tensor, sinoT: torch.Tensor,
     return out, err[:num_iters], xerr[:num_iters], st
 
 
+def graph_replays() -> int:
+    """MBIR attempts this process replayed from HIP graphs."""
+    c = C.c_long()
+    _lib.call("tc_graph_replays", C.byref(c))
+    return c.value
+
+
 def ktimer_enable(kernel_class: int, on: bool = True) -> None:
     _lib.call("tc_ktimer_enable", int(kernel_class), int(bool(on)))
 
