@@ -15,19 +15,28 @@ namespace tc {
 // lines per workgroup of the compile-time plan for FFT length M (0: none)
 int toeplitz_ct_lpw(int M);
 extern const int kToeplitzCtSizes[];
+extern const float kToeplitzCtCost[];
 extern const int kToeplitzCtCount;
 
 // FFT length for an n x n Toeplitz product: any even length >= 2n - 1 gives
-// the same result; take the smallest compile-time plan size >= 2n - 1 when
-// it is within 10 % of the smallest even 2,3,5,7-smooth length, else that.
+// the same result.  Among the compile-time plan sizes >= 2n - 1 within 10 %
+// of the smallest even 2,3,5,7-smooth length, take the one with the least
+// size x relative cost per point; with none, that smooth length.
 int toeplitz_fft_length(int n) {
   const int m = 2 * n - 1;
   const int smooth = fft_next_smooth(m, true);
+  int best = 0;
+  float best_cost = 0.f;
   for (int i = 0; i < kToeplitzCtCount; ++i) {
     const int c = kToeplitzCtSizes[i];
-    if (c >= m) return c <= smooth + smooth / 10 ? c : smooth;
+    if (c < m || c > smooth + smooth / 10) continue;
+    const float cost = (float)c * kToeplitzCtCost[i];
+    if (!best || cost < best_cost) {
+      best = c;
+      best_cost = cost;
+    }
   }
-  return smooth;
+  return best ? best : smooth;
 }
 
 // ---------------------------------------------------------------------------

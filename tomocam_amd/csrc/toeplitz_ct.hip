@@ -28,7 +28,10 @@ struct Xform<L, E, DIR, Radices<Rs...>> {
   __device__ __forceinline__ static void run(float2* v, float2* line, int t,
                                              bool active,
                                              const float2* __restrict__ tw) {
-    ct::transform<L, E, DIR, ZIN, Rs...>(v, line, t, active, tw);
+    // the zero-upper DFT form needs a radix-2 first factor; other first
+    // radices still skip the padding's loads and compute on zeros
+    constexpr bool z = ZIN && ct::first_radix<first>() == 2;
+    ct::transform<L, E, DIR, z, Rs...>(v, line, t, active, tw);
   }
 };
 
@@ -71,6 +74,7 @@ using Plan2250 = TPlan<2250, 30, 3, Radices<10, 15, 15>, Radices<15, 15, 10>>;
 using Plan2916 = TPlan<2916, TC_P2916_E, TC_P2916_LPW, Radices<18, 18, 9>,
                        Radices<9, 18, 18>, TC_P2916_LPWC, false>;
 using Plan4500 = TPlan<4500, 30, 2, Radices<10, 15, 30>, Radices<30, 15, 10>>;
+using Plan4608 = TPlan<4608, 24, 2, Radices<24, 24, 8>, Radices<8, 24, 24>, 1>;
 using Plan5832 = TPlan<5832, 18, 1, Radices<18, 18, 18>, Radices<18, 18, 18>>;
 
 // Wt element (slice s, column k1, row): layout [s][k1][rp], so the column
@@ -675,8 +679,14 @@ static void run_ct(const void* x, const float* b, float* out, void* wt,
 }
 
 // FFT lengths with compile-time plans, ascending
-extern const int kToeplitzCtSizes[] = {750, 1458, 2250, 2916, 4500, 5832};
-extern const int kToeplitzCtCount = 6;
+extern const int kToeplitzCtSizes[] = {750, 1458, 2250, 2916, 4500, 4608,
+                                       5832};
+// relative time per point of each plan: the 10 * 15 * 30 plan for 4500
+// (radix-5 factors, 168 VGPRs in the column pass) measured 1.54x the
+// 24 * 24 * 8 plan for 4608 per point at n = 2180 (column pass 11.7 against
+// 7.8 ms over 20 iterations)
+extern const float kToeplitzCtCost[] = {1.f, 1.f, 1.f, 1.f, 1.54f, 1.f, 1.f};
+extern const int kToeplitzCtCount = 7;
 
 // column-pass lines per workgroup (0: no compile-time plan for M)
 int toeplitz_ct_lpw(int M) {
@@ -686,6 +696,7 @@ int toeplitz_ct_lpw(int M) {
     case 2250: return Plan2250::LPWC;
     case 2916: return Plan2916::LPWC;
     case 4500: return Plan4500::LPWC;
+    case 4608: return Plan4608::LPWC;
     case 5832: return Plan5832::LPWC;
     default: return 0;
   }
@@ -708,6 +719,7 @@ static bool apply_ct(int M, const void* x, const float* b, float* out,
     case 2250: TC_RUN_CT(Plan2250);
     case 2916: TC_RUN_CT(Plan2916);
     case 4500: TC_RUN_CT(Plan4500);
+    case 4608: TC_RUN_CT(Plan4608);
     case 5832: TC_RUN_CT(Plan5832);
     default: return false;
   }
