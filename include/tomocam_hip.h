@@ -322,7 +322,10 @@ int tc_ktimer_collect(int kernel_class, double* total_ms, long* count,
  * buffer role and replay it, except with kernel timing enabled, an FFT
  * length without compile-time passes, the separate commit kernel, a second
  * Toeplitz stream, or concurrent tc_mbir2_batch workers.  Off by default:
- * it measured no faster than launching the kernels. */
+ * it measured no faster than launching the kernels.  While one thread
+ * captures, a synchronous HIP call on the null stream from another thread
+ * of the process fails, so turn it on only where MBIR calls are the
+ * process's only concurrent HIP work. */
 int tc_graph_replays(long* count);
 
 #ifdef __cplusplus
