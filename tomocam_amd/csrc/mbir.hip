@@ -566,9 +566,19 @@ __global__ void reduce_dots_kernel(const double* __restrict__ q, long qcount,
   const double* p = k < nq ? q + k : t;
   const long count = k < nq ? qcount : tcount;
   const int stride = k < nq ? nq : 1;
-  double acc = 0.0;
-  for (long i = threadIdx.x; i < count; i += blockDim.x)
-    acc += p[i * stride];
+  // eight independent partial sums keep eight loads in flight per thread
+  // (one block sums tens of thousands of Toeplitz partials); fixed order.
+  // 22.7 -> 12.5 us per attempt at 32 x 1454^2.
+  const long bd = blockDim.x;
+  double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  long i = threadIdx.x;
+  for (; i + 7 * bd < count; i += 8 * bd) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] += p[(i + u * bd) * stride];
+  }
+  for (; i < count; i += bd) a[0] += p[i * stride];
+  const double acc = ((a[0] + a[1]) + (a[2] + a[3])) +
+                     ((a[4] + a[5]) + (a[6] + a[7]));
   double tot = block_sum(acc, red);
   if (threadIdx.x == 0) out[k] = tot;
 }
