@@ -38,7 +38,7 @@ def _sino(N, P, S, seed, noise=0.01):
 @pytest.mark.parametrize("n,S", [(362, 2), (726, 2), (1090, 1), (1454, 2),
                                  (2180, 1)])
 def test_toeplitz_mixed_sizes(torch_cuda, n, S):
-    """FFT lengths 750, 1458, 2250, 2916, 4608 (the polar-grid PSF at 362, a
+    """FFT lengths 750, 1458, 2304, 2916, 4608 (the polar-grid PSF at 362, a
     synthetic even PSF above): random and piecewise-constant inputs, and an
     all-zero slice (scale 1, exact zeros out)."""
     torch = torch_cuda
