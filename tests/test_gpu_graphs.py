@@ -22,7 +22,7 @@ from conftest import announce
 
 pytestmark = pytest.mark.gpu
 
-N, P, S = 256, 180, 3   # padded width 362, FFT length 750 (compile-time passes)
+N, P, S = 256, 180, 3   # padded width 362, FFT length 768 (compile-time passes)
 
 
 def _timers_off(gpu):
@@ -49,7 +49,7 @@ def problem(torch_cuda):
     torch = torch_cuda
     from tomocam_amd import gpu
     psf = gpu.PointSpreadFunction(PH.angles(P), 362)
-    assert psf.fft_size == 750
+    assert psf.fft_size == 768
     xt = np.stack([PH.random_ellipses(362, 71 + s) for s in range(S)])
     xt = torch.tensor(xt, dtype=torch.float32).cuda()
     b = psf.convolve(xt)

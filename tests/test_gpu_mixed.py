@@ -38,7 +38,7 @@ def _sino(N, P, S, seed, noise=0.01):
 @pytest.mark.parametrize("n,S", [(362, 2), (726, 2), (1090, 1), (1454, 2),
                                  (2180, 1)])
 def test_toeplitz_mixed_sizes(torch_cuda, n, S):
-    """FFT lengths 750, 1458, 2304, 2916, 4608 (the polar-grid PSF at 362, a
+    """FFT lengths 768, 1458, 2304, 2916, 4608 (the polar-grid PSF at 362, a
     synthetic even PSF above): random and piecewise-constant inputs, and an
     all-zero slice (scale 1, exact zeros out)."""
     torch = torch_cuda
@@ -87,7 +87,7 @@ def test_toeplitz_mixed_needs_compile_time_plan(torch_cuda):
 
 
 def test_mbir2_mixed_vs_oracle(torch_cuda):
-    """4 x 256^2 (padded 362, FFT length 750), 180 angles, 100 iterations,
+    """4 x 256^2 (padded 362, FFT length 768), 180 angles, 100 iterations,
     sigma = 100: both precisions against the FP64 oracle."""
     torch = torch_cuda
     from tomocam_amd import gpu

@@ -172,7 +172,7 @@ def test_run2_fused_commit_with_retries(torch_cuda, n, S, mult, retries):
 def test_run2_slice_groups_and_two_streams(torch_cuda, monkeypatch):
     """The compile-time passes in several slice groups (5 slices in groups of
     2: 2 + 2 + 1), on one stream and alternating over two, against one
-    group and against the FP64 oracle (padded width 362, FFT length 750)."""
+    group and against the FP64 oracle (padded width 362, FFT length 768)."""
     torch = torch_cuda
     ang = PH.angles(60).astype(np.float64)
     img = np.stack([PH.random_ellipses(256, 500 + s) for s in range(5)])

@@ -45,7 +45,7 @@ def test_toeplitz_polar_psf(torch_cuda, n, P, S):
 @pytest.mark.parametrize("n,S", [(181, 2), (362, 2), (726, 1), (1090, 2),
                                  (1454, 3), (2180, 1), (2908, 1)])
 def test_toeplitz_sizes(torch_cuda, n, S):
-    """FFT sizes of the workloads: M = 378 (generic kernels), 750, 1458,
+    """FFT sizes of the workloads: M = 378 (generic kernels), 768, 1458,
     2304, 2916, 4608, 5832 (compile-time plans)."""
     torch = torch_cuda
     from tomocam_amd import gpu

@@ -7,16 +7,17 @@ import math
 import pytest
 
 
-@pytest.mark.parametrize("n,M", [(362, 750), (726, 1458), (1090, 2304),
+@pytest.mark.parametrize("n,M", [(362, 768), (340, 750), (726, 1458),
+                                 (1090, 2304),
                                  (1454, 2916), (2180, 4608), (2908, 5832),
                                  (2048, 4500), (2250, 4608), (2304, 4608),
                                  (1024, 2250), (1152, 2304),
                                  (181, 378), (44, 90), (2, 4)])
 def test_toeplitz_fft_length(n, M):
     """M >= 2n - 1; the compile-time plan sizes for the workload widths; where
-    both 4500 and 4608 (or 2250 and 2304) fit, the larger (its plan is 1.54x
-    (1.17x) cheaper per point); the smallest even 2,3,5,7-smooth length where
-    no plan is within 10 %."""
+    both 4500 and 4608 (2250 and 2304, 750 and 768) fit, the larger (its plan
+    is 1.54x (1.17x, 1.22x) cheaper per point); the smallest even
+    2,3,5,7-smooth length where no plan is within 10 %."""
     from tomocam_amd import _lib
     m = C.c_int()
     _lib.call("tc_toeplitz_fft_length", n, C.byref(m))

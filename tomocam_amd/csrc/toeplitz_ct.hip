@@ -76,6 +76,7 @@ using Plan2916 = TPlan<2916, TC_P2916_E, TC_P2916_LPW, Radices<18, 18, 9>,
 using Plan4500 = TPlan<4500, 30, 2, Radices<10, 15, 30>, Radices<30, 15, 10>>;
 using Plan4608 = TPlan<4608, 24, 2, Radices<24, 24, 8>, Radices<8, 24, 24>, 1>;
 using Plan2304 = TPlan<2304, 24, 2, Radices<24, 24, 4>, Radices<4, 24, 24>, 2>;
+using Plan768 = TPlan<768, 24, 6, Radices<24, 8, 4>, Radices<4, 8, 24>, 6>;
 using Plan5832 = TPlan<5832, 18, 1, Radices<18, 18, 18>, Radices<18, 18, 18>>;
 
 // Wt element (slice s, column k1, row): layout [s][k1][rp], so the column
@@ -680,16 +681,18 @@ static void run_ct(const void* x, const float* b, float* out, void* wt,
 }
 
 // FFT lengths with compile-time plans, ascending
-extern const int kToeplitzCtSizes[] = {750, 1458, 2250, 2304, 2916, 4500,
-                                       4608, 5832};
+extern const int kToeplitzCtSizes[] = {750, 768, 1458, 2250, 2304, 2916,
+                                       4500, 4608, 5832};
 // relative time per point of each plan, where two can serve one width:
 // the 10 * 15 * 30 plan for 4500 (radix-5 factors, 168 VGPRs in the column
 // pass) measured 1.54x the 24 * 24 * 8 plan for 4608 per point at n = 2180
 // (column pass 11.7 against 7.8 ms over 20 iterations); the 10 * 15 * 15
 // plan for 2250 measured 1.17x the 24 * 24 * 4 plan for 2304 at n = 1090
-// (3.87 against 3.39 ms)
-extern const float kToeplitzCtCost[] = {1.f, 1.f, 1.17f, 1.f, 1.f, 1.54f, 1.f, 1.f};
-extern const int kToeplitzCtCount = 8;
+// (3.87 against 3.39 ms); the 10 * 15 * 5 plan for 750 1.22x the 24 * 8 * 4
+// plan for 768 at n = 362 (the three passes 2.98 against 2.49 ms)
+extern const float kToeplitzCtCost[] = {1.22f, 1.f,  1.f, 1.17f, 1.f,
+                                        1.f,   1.54f, 1.f, 1.f};
+extern const int kToeplitzCtCount = 9;
 
 // column-pass lines per workgroup (0: no compile-time plan for M)
 int toeplitz_ct_lpw(int M) {
@@ -701,6 +704,7 @@ int toeplitz_ct_lpw(int M) {
     case 4500: return Plan4500::LPWC;
     case 4608: return Plan4608::LPWC;
     case 2304: return Plan2304::LPWC;
+    case 768: return Plan768::LPWC;
     case 5832: return Plan5832::LPWC;
     default: return 0;
   }
@@ -725,6 +729,7 @@ static bool apply_ct(int M, const void* x, const float* b, float* out,
     case 4500: TC_RUN_CT(Plan4500);
     case 4608: TC_RUN_CT(Plan4608);
     case 2304: TC_RUN_CT(Plan2304);
+    case 768: TC_RUN_CT(Plan768);
     case 5832: TC_RUN_CT(Plan5832);
     default: return false;
   }
