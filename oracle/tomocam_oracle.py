@@ -640,6 +640,9 @@ class Run2Log:
     xerrs: List[float] = field(default_factory=list)
     backtracks: int = 0
     steps: List[float] = field(default_factory=list)
+    # optional per-attempt trace (run2 only): set to a list to record, for
+    # every attempt, the scalars and arrays the control flow used
+    attempts: Optional[List[dict]] = None
 
 
 def run2(x0: np.ndarray, grad_fn: Callable, err_fn: Callable, num_iters: int,
@@ -656,10 +659,15 @@ def run2(x0: np.ndarray, grad_fn: Callable, err_fn: Callable, num_iters: int,
             tnew = 0.5 * (math.sqrt(t ** 4 + 4.0 * t ** 2) - t ** 2)
             y = sol + (sol - x) * beta
             g = grad_fn(y)
+            sol_prev = sol
             sol = y - g * step
             fx = err_fn(sol)
             fy = err_fn(y)
             gy = 0.5 * step * float(np.sum(g * g))
+            if log is not None and log.attempts is not None:
+                log.attempts.append(dict(
+                    t=t, tnew=tnew, beta=beta, step=step, sol_prev=sol_prev,
+                    x=x, y=y, g=g, cand=sol, accepted=not (fx > fy + gy)))
             if fx > fy + gy:
                 step *= 0.9
                 if log is not None:

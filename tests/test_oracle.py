@@ -285,6 +285,140 @@ def test_run2_retry_semantics():
     assert all(s < 6.0 for s in log.steps[:1])
 
 
+def _odd_setup():
+    """Odd widths (15 -> padded 21), where the projector-path error
+    ||P v - y||^2 equals the Toeplitz expansion exactly, so the accept test
+    can be judged without the oracle's own function_value2."""
+    P, N, S = 12, 15, 2
+    ang = np.arange(P) * np.pi / P
+    rng = np.random.default_rng(5)
+    img = rng.random((S, N, N))
+    sino = O.project(img, ang, exact=True)
+    return O.mbir2_setup(sino, ang, N / 2, exact=True), ang
+
+
+def _run2_traced(st, x0, sigma, step0, iters, err_sq=False):
+    reg = lambda v: O.qggmrf_grad(v, sigma, 1.2)
+
+    def grad_fn(x):
+        return O.gradient2(x, st.sinoT, st.psf) + reg(x)
+
+    def err_fn(x):
+        f = O.function_value2(x, st.sinoT, st.psf, st.sino_norm)
+        return f if err_sq else math.sqrt(f)
+
+    log = O.Run2Log()
+    log.attempts = []
+    out = O.run2(x0, grad_fn, err_fn, iters, step0, log)
+    return out, log, reg
+
+
+def _run2_violations(st, ang, x0, step0, iters, log, reg):
+    """Checks the trace of O.run2 against src/optimize.h:96-154 with
+    quantities computed on other paths than the oracle's run2 uses:
+      - f(v) = ||P v - y||^2 by the exact projector (function_value), and
+        the accept test sqrt f(cand) <= sqrt f(y) + s/2 sum g^2
+        (src/optimize.h:123-127: a norm against a squared norm);
+      - g = P^T (P y - y) + R'(y) by projector and backprojector;
+      - tnew^2 = t^2 (1 - tnew) (the root the t-update takes, :110-112);
+      - beta = tnew_prev (1/t - 1), y = sol + beta (sol - x) (:109-116);
+      - the step is step0 on an iteration's first attempt and shrinks by
+        0.9 per rejection (:128-132); the momentum term vanishes on the
+        first attempt after an acceptance (x = sol there, :134);
+      - exactly `iters` acceptances (tol/xtol unused).
+    Returns a list of messages (empty when every check holds)."""
+    bad = []
+    fp = lambda v: O.function_value(v, st.sino2, ang, exact=True)
+    x_acc = np.asarray(x0, dtype=np.float64)
+    sol = x_acc
+    t_exp, tnew_prev = 1.0, 1.0
+    step_exp, first = step0, True
+    for k, a in enumerate(log.attempts):
+        t, tn, beta, s = a["t"], a["tnew"], a["beta"], a["step"]
+        if abs(t - t_exp) > 1e-15:
+            bad.append(f"{k}: t {t} != last accepted tnew {t_exp}")
+        if abs(tn * tn - t * t * (1.0 - tn)) > 1e-14:
+            bad.append(f"{k}: tnew^2 != t^2 (1 - tnew)")
+        if abs(beta - tnew_prev * (1.0 / t - 1.0)) > 1e-15:
+            bad.append(f"{k}: beta != tnew_prev (1/t - 1)")
+        if s != step_exp:
+            bad.append(f"{k}: step {s} != {step_exp}")
+        if not np.array_equal(a["x"], x_acc) or not np.array_equal(
+                a["sol_prev"], sol):
+            bad.append(f"{k}: x / sol not the last accepted / candidate")
+        y_exp = sol + beta * (sol - x_acc)
+        if np.abs(a["y"] - y_exp).max() > 1e-15 * np.abs(y_exp).max():
+            bad.append(f"{k}: y != sol + beta (sol - x)")
+        if first and not np.array_equal(a["y"], x_acc):
+            bad.append(f"{k}: momentum term non-zero after an acceptance")
+        g_ind = O.backproject(O.project(a["y"], ang, exact=True) - st.sino2,
+                              ang, exact=True) + reg(a["y"])
+        if np.abs(a["g"] - g_ind).max() > 1e-10 * np.abs(g_ind).max():
+            bad.append(f"{k}: g != P^T (P y - y) + R'(y)")
+        cand = a["y"] - s * g_ind
+        fx, fy = math.sqrt(fp(cand)), math.sqrt(fp(a["y"]))
+        gy = 0.5 * s * float(np.sum(g_ind * g_ind))
+        margin = (fx - fy - gy) / (fy + gy)
+        if abs(margin) < 1e-9:
+            bad.append(f"{k}: accept test too close to call ({margin:.1e})")
+        elif a["accepted"] != (fx <= fy + gy):
+            bad.append(f"{k}: accepted={a['accepted']} but sqrt f(cand) - "
+                       f"sqrt f(y) - s/2 |g|^2 = {margin:.3e} (relative)")
+        tnew_prev = tn
+        sol = a["cand"]
+        if a["accepted"]:
+            x_acc, t_exp, step_exp, first = sol, tn, step0, True
+        else:
+            step_exp, first = s * 0.9, False
+    n_acc = sum(a["accepted"] for a in log.attempts)
+    if n_acc != iters:
+        bad.append(f"{n_acc} acceptances, expected {iters}")
+    return bad
+
+
+def test_run2_control_flow_pinned_by_projector_path():
+    """sigma = 10 from the FBP guess: 15 accepted attempts, every one
+    checked against projector-path errors and gradients."""
+    st, ang = _odd_setup()
+    _, log, reg = _run2_traced(st, st.x0, 10.0, st.step, 15)
+    assert len(log.attempts) == 15
+    assert _run2_violations(st, ang, st.x0, st.step, 15, log, reg) == []
+
+
+def _backtracking_start(st):
+    T = lambda v: O.toeplitz(v, st.psf)
+    reg = lambda v: O.qggmrf_grad(v, 1e4, 1.2)
+    xs = O.run2_linear(st.x0, T, st.sinoT, reg, st.sino_norm, 150, 3.0)
+    return xs + 1e-3
+
+
+def test_run2_backtracking_pinned_by_projector_path():
+    """A perturbed converged point with step 6 (sigma = 1e4): rejections
+    with and without momentum, each rejected attempt violating the
+    projector-path accept test and each accepted one satisfying it."""
+    st, ang = _odd_setup()
+    x0 = _backtracking_start(st)
+    _, log, reg = _run2_traced(st, x0, 1e4, 6.0, 2)
+    acc = [a["accepted"] for a in log.attempts]
+    assert acc.count(False) >= 4 and acc.count(True) == 2
+    # a retry with non-zero momentum (sol != x after a rejection)
+    assert any(a["beta"] != 0.0 and not np.array_equal(a["sol_prev"], a["x"])
+               for a in log.attempts)
+    assert _run2_violations(st, ang, x0, 6.0, 2, log, reg) == []
+
+
+def test_run2_pin_detects_squared_norm_accept_test():
+    """The checks above fail for a plausible mistake planted in the
+    oracle: the accept test on squared norms f(cand) > f(y) + s/2 |g|^2
+    (what a 'fixed' reference would do) takes decisions the reference's
+    norm-against-squared-norm test does not."""
+    st, ang = _odd_setup()
+    x0 = _backtracking_start(st)
+    _, log, reg = _run2_traced(st, x0, 1e4, 6.0, 2, err_sq=True)
+    bad = _run2_violations(st, ang, x0, 6.0, 2, log, reg)
+    assert any("accepted=" in m for m in bad), bad
+
+
 # ---------------------------------------------------------------- pre/post
 
 
