@@ -328,6 +328,18 @@ int tc_ktimer_collect(int kernel_class, double* total_ms, long* count,
  * process's only concurrent HIP work. */
 int tc_graph_replays(long* count);
 
+/* Device-memory arenas (no counterpart in the reference, whose every call
+ * allocates and frees its temporaries).  The MBIR drivers take their
+ * buffers from a per-host-thread cache, so a repeated call allocates
+ * nothing and never waits in hipFree.  Device allocations made for the
+ * caches so far, requests served from a cache, and the bytes cached. */
+int tc_alloc_stats(long* device_mallocs, long* cache_hits,
+                   double* cached_bytes);
+
+/* Frees the cached device memory of the calling thread and of every cache
+ * no thread is using. */
+int tc_release_cached(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -35,12 +35,15 @@ def _sino(N, P, S, seed, noise=0.01):
     return ang, sino
 
 
-@pytest.mark.parametrize("n,S", [(362, 2), (726, 2), (1090, 1), (1454, 2),
-                                 (2180, 1)])
-def test_toeplitz_mixed_sizes(torch_cuda, n, S):
-    """FFT lengths 768, 1458, 2304, 2916, 4608 (the polar-grid PSF at 362, a
-    synthetic even PSF above): random and piecewise-constant inputs, and an
-    all-zero slice (scale 1, exact zeros out)."""
+@pytest.mark.parametrize("n,S,M", [(340, 2, 750), (362, 2, 768),
+                                   (726, 2, 1458), (1024, 1, 2250),
+                                   (1090, 1, 2304), (1454, 2, 2916),
+                                   (2048, 1, 4500), (2180, 1, 4608)])
+def test_toeplitz_mixed_sizes(torch_cuda, n, S, M):
+    """FFT lengths 750, 768, 1458, 2250, 2304, 2916, 4500, 4608 (the
+    polar-grid PSF up to 362, a synthetic even PSF above): random and
+    piecewise-constant inputs, and an all-zero slice (scale 1, exact zeros
+    out)."""
     torch = torch_cuda
     from tomocam_amd import gpu
     ang = PH.angles(180)
@@ -50,6 +53,7 @@ def test_toeplitz_mixed_sizes(torch_cuda, n, S):
         ps = 0.5 * (r + r[::-1, ::-1])
     psf = gpu.PointSpreadFunction(
         psf_array=torch.tensor(ps, dtype=torch.float32).cuda())
+    assert psf.fft_size == M
     rng = np.random.default_rng(n + 3)
     for kind in ("random", "ellipses"):
         if kind == "random":

@@ -42,18 +42,23 @@ def test_toeplitz_polar_psf(torch_cuda, n, P, S):
     assert abs(xtx - np.sum(x * ref)) <= 1e-5 * abs(np.sum(x * ref))
 
 
-@pytest.mark.parametrize("n,S", [(181, 2), (362, 2), (726, 1), (1090, 2),
-                                 (1454, 3), (2180, 1), (2908, 1)])
-def test_toeplitz_sizes(torch_cuda, n, S):
-    """FFT sizes of the workloads: M = 378 (generic kernels), 768, 1458,
-    2304, 2916, 4608, 5832 (compile-time plans)."""
+@pytest.mark.parametrize("n,S,M", [(181, 2, 378), (340, 2, 750),
+                                   (362, 2, 768), (726, 1, 1458),
+                                   (1024, 2, 2250), (1090, 2, 2304),
+                                   (1454, 3, 2916), (2048, 1, 4500),
+                                   (2180, 1, 4608), (2908, 1, 5832)])
+def test_toeplitz_sizes(torch_cuda, n, S, M):
+    """FFT length M of each compile-time plan (750, 768, 1458, 2250, 2304,
+    2916, 4500, 4608, 5832) and of the generic kernels (378), pinned by
+    the length rule's choice for the width."""
     torch = torch_cuda
     from tomocam_amd import gpu
     ps = _even_psf(n, 7)
     x = np.random.default_rng(n + 1).standard_normal((S, n, n))
     ref = O.toeplitz(x, ps)
-    announce(f"toeplitz size n={n} S={S}")
+    announce(f"toeplitz size n={n} S={S} M={M}")
     psf = gpu.PointSpreadFunction(psf_array=torch.tensor(ps, dtype=torch.float32).cuda())
+    assert psf.fft_size == M
     out, xtx = psf.convolve(torch.tensor(x, dtype=torch.float32).cuda(),
                             with_dot=True)
     assert rel_l2(out.cpu().numpy(), ref) < 2e-5

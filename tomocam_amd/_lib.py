@@ -74,6 +74,8 @@ SIGNATURES = {
                                    C.c_int, vp],
     "tc_mbir2_host": [vp, vp, C.c_int, C.c_int, C.c_int, vp, C.c_float,
                       C.c_int, C.c_float, C.c_float, C.c_float, C.c_int],
+    "tc_alloc_stats": [c_long_p, c_long_p, c_double_p],
+    "tc_release_cached": [],
     "tc_ktimer_enable": [C.c_int, C.c_int],
     "tc_ktimer_collect": [C.c_int, c_double_p, c_long_p, c_double_p],
 }
@@ -88,7 +90,9 @@ class TomocamError(RuntimeError):
 
 
 def lib_path() -> str:
-    return os.path.join(_HERE, LIBNAME)
+    """The in-tree build, or the file named by TOMOCAM_HIP_LIB (another
+    build of the same library, e.g. with different compile-time options)."""
+    return os.environ.get("TOMOCAM_HIP_LIB") or os.path.join(_HERE, LIBNAME)
 
 
 def lib():

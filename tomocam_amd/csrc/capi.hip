@@ -585,6 +585,17 @@ int tc_graph_replays(long* count) {
   });
 }
 
+int tc_alloc_stats(long* device_mallocs, long* cache_hits,
+                   double* cached_bytes) {
+  return guarded([&] {
+    tc::arena_stats(device_mallocs, cache_hits, cached_bytes);
+  });
+}
+
+int tc_release_cached(void) {
+  return guarded([&] { tc::arena_trim(); });
+}
+
 int tc_ktimer_collect(int kernel_class, double* total_ms, long* count,
                       double* bytes) {
   return guarded([&] {

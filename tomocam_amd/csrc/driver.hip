@@ -97,6 +97,9 @@ Mbir2Result mbir2(const float* sino, const float* x0, float* recon, int S,
                   const Mbir2Options& o, hipStream_t stream) {
   TC_REQUIRE(S >= 1 && P >= 1 && N >= 2, "mbir2: bad sinogram shape");
   TC_REQUIRE(h_angles != nullptr, "mbir2: null angles");
+  // the call's buffers come from (and go back to) the thread's arena; the
+  // call ends with a stream synchronisation
+  ArenaScope arena;
   Mbir2Result res;
   const long nsino = (long)S * P * N;
 
@@ -270,6 +273,7 @@ Mbir2Result mbir_nufft(const float* sino, const float* x0, float* recon, int S,
   TC_REQUIRE(S >= 1 && P >= 1 && N >= 2, "mbir: bad sinogram shape");
   TC_REQUIRE(h_angles != nullptr && x0 != nullptr,
              "mbir: angles and an initial guess are required");
+  ArenaScope arena;
   Mbir2Result res;
   const long nsino = (long)S * P * N;
   const size_t img_bytes = sizeof(float) * (size_t)S * N * N;

@@ -221,30 +221,112 @@ __device__ __forceinline__ void stage_compute(float2* v, int t,
   }
 }
 
-// Writes stage outputs (radix R, Ns) to LDS at their Stockham destinations.
-template <int L, int T, int E, int R, int Ns>
+// LDS layout of one exchange (stage of radix Rp, Ns = Nsp, writing; stage
+// of radix R reading).  The writer's outputs form blocks of D = Nsp Rp
+// consecutive elements; every M = K D elements the layout inserts P unused
+// slots: index i lives at i + (i / M) P.  With 64 dword banks, a wave's
+// float2 accesses are conflict-free when its lanes' addresses are distinct
+// modulo 32:
+//   Nsp = 1: lane j writes j Rp + r.  An odd Rp needs nothing; an even Rp
+//            gets one slot per K lanes, K = 32 / gcd(Rp, 32) (or the
+//            largest K the reader allows), which staggers the lanes that
+//            would share a bank;
+//   Nsp > 1: lane j writes (j / Nsp)(D + P) + j mod Nsp + r Nsp, which is
+//            j modulo 32 when D + P = Nsp (mod 32) (K = 1); where that P
+//            would take more than a tenth of the line, modulo 16, else 0.
+// The reader loads c + r L/R, and L/R is a multiple of M, so on both sides
+// the padded address is a per-thread base plus a compile-time offset.
+// TC_LDS_PAD=0 builds the plain Stockham layout (no padding).
+#ifndef TC_LDS_PAD
+#define TC_LDS_PAD 1
+#endif
+struct XLayout {
+  int M;  // elements per padded block
+  int P;  // unused slots after each block
+};
+constexpr int gcd_c(int a, int b) {
+  while (b) {
+    const int t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+constexpr XLayout xchg_layout(int L, int Nsp, int Rp, int R) {
+  const int D = Nsp * Rp;
+  if (!TC_LDS_PAD) return {D, 0};
+  if (Nsp == 1) {
+    if (Rp & 1) return {D, 0};
+    const int want = 32 / gcd_c(Rp, 32);
+    const int q = (L / R) / D;  // K must divide the reader's stride / D
+    int K = 1;
+    for (int k = 1; k <= want; ++k)
+      if (q % k == 0) K = k;
+    return {D * K, 1};
+  }
+  // at most ~10 % of the line in padding: else the weaker 16-slot
+  // condition (32 banks), else none
+  for (int mod : {32, 16}) {
+    const int P = (((Nsp - D) % mod) + mod) % mod;
+    if (((L - 1) / D) * P * 10 <= L) return {D, P};
+  }
+  return {D, 0};
+}
+
+// Writes the outputs of a stage of radix Rp (Ns = Nsp) to LDS at their
+// Stockham destinations, in the layout of the exchange into radix R.
+template <int L, int T, int E, int Rp, int Nsp, int R>
 __device__ __forceinline__ void stage_write(const float2* v, float2* line,
                                             int t) {
-  constexpr int NB = E / R;
+  constexpr int NB = E / Rp;
+  constexpr int D = Nsp * Rp;
+  constexpr XLayout X = xchg_layout(L, Nsp, Rp, R);
+  constexpr int K = X.M / D;
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
     const int j = t + i * T;
-    const int k = j % Ns;
-    const int base = (j - k) * R + k;
+    const int k = j % Nsp;
+    const int blk = j / Nsp;  // output block of D elements
+    // (blk D + k + r Nsp) / M = blk / K, since k + r Nsp < D
+    const int base = blk * D + k + (X.P ? (blk / K) * X.P : 0);
 #pragma unroll
-    for (int r = 0; r < R; ++r) line[base + r * Ns] = v[i * R + r];
+    for (int r = 0; r < Rp; ++r) line[base + r * Nsp] = v[i * Rp + r];
   }
 }
 
-// Reads the inputs of a radix-R stage from LDS.
-template <int L, int T, int E, int R>
+// Reads the inputs of a radix-R stage from the exchange written by
+// stage_write<L, T, E, Rp, Nsp, R>.
+template <int L, int T, int E, int R, int Nsp, int Rp>
 __device__ __forceinline__ void stage_read(float2* v, const float2* line,
                                            int t) {
   constexpr int NB = E / R;
+  constexpr XLayout X = xchg_layout(L, Nsp, Rp, R);
+  constexpr int Q = L / R;  // input stride, a multiple of M
+  static_assert(Q % X.M == 0, "exchange block must divide the input stride");
+  constexpr int QP = Q + (Q / X.M) * X.P;
 #pragma unroll
-  for (int i = 0; i < NB; ++i)
+  for (int i = 0; i < NB; ++i) {
+    const int c = t + i * T;
+    const int base = c + (X.P ? (c / X.M) * X.P : 0);
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[i * R + r] = line[t + i * T + r * (L / R)];
+    for (int r = 0; r < R; ++r) v[i * R + r] = line[base + r * QP];
+  }
+}
+
+// LDS elements one line needs: the largest padded index of any exchange of
+// the plan, plus one; at least L for the natural-order uses outside the
+// exchanges.
+template <int L, int... Rs>
+constexpr int line_elems() {
+  const int a[] = {Rs...};
+  int need = L, nsp = 1;
+  for (int s = 1; s < (int)sizeof...(Rs); ++s) {
+    const XLayout X = xchg_layout(L, nsp, a[s - 1], a[s]);
+    const int e = L + ((L - 1) / X.M) * X.P;
+    if (e > need) need = e;
+    nsp *= a[s - 1];
+  }
+  return need;
 }
 
 // Runs stages s..S-1 of the plan (radices Rs in order): stage s computes on
@@ -263,9 +345,9 @@ struct StagesDispatch {
       if constexpr (s > 0) {
         constexpr int Rp = RL::at(s - 1);
         constexpr int Nsp = RL::ns(s - 1);
-        if (active) stage_write<L, T, E, Rp, Nsp>(v, line, t);
+        if (active) stage_write<L, T, E, Rp, Nsp, R>(v, line, t);
         __syncthreads();
-        if (active) stage_read<L, T, E, R>(v, line, t);
+        if (active) stage_read<L, T, E, R, Nsp, Rp>(v, line, t);
         __syncthreads();
       }
       if (active) stage_compute<L, T, E, R, Ns, DIR, ZIN && s == 0>(v, t, tw);

@@ -758,6 +758,8 @@ Run2Stats run2_device(Psf& psf, const float* x0, const float* b, float* out,
   TC_REQUIRE(opt.num_iters >= 0, "run2: num_iters must be >= 0");
   const long N = (long)S * n * n;
   const size_t vb = sizeof(float) * (size_t)N;
+  // buffers from the thread's arena (the loop ends with a synchronisation)
+  ArenaScope arena;
   DevBuf bA(vb), bB(vb), bTA(vb), bTB(vb), bg(vb), bTg(vb);
   float* A = bA.as<float>();
   float* B = bB.as<float>();
@@ -784,12 +786,8 @@ Run2Stats run2_device(Psf& psf, const float* x0, const float* b, float* out,
   // the next one's), per-wave maxima, saturation counter
   DevBuf bg16, bgsc, bgmax, bsat;
   DevBuf dots_dev(sizeof(double) * 8);
-  double* dots_h = nullptr;
-  TC_HIP(hipHostMalloc((void**)&dots_h, sizeof(double) * 8, hipHostMallocDefault));
-  struct HostFree {
-    double* p;
-    ~HostFree() { if (p) (void)hipHostFree(p); }
-  } hf{dots_h};
+  // pinned landing area of the per-attempt dots (the arena's, kept with it)
+  double* dots_h = arena_pinned_dots();
   // TOMOCAM_HIP_FUSED_COMMIT=0 selects the separate commit kernel (same
   // arithmetic, one more pass over HBM)
   const char* fenv = std::getenv("TOMOCAM_HIP_FUSED_COMMIT");

@@ -1,4 +1,8 @@
-// Error strings and optional per-kernel-class event timing.
+// Error strings, the per-thread device-memory arenas and optional
+// per-kernel-class event timing.
+#include <algorithm>
+#include <atomic>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -10,6 +14,167 @@ static thread_local std::string g_last_error;
 
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 const char* last_error() { return g_last_error.c_str(); }
+
+// ---------------------------------------------------------------------------
+// arenas
+// ---------------------------------------------------------------------------
+
+struct Arena {
+  std::mutex mu;
+  std::multimap<size_t, void*> free;  // capacity -> block
+  size_t free_bytes = 0;
+  int dev = -1;
+  double* pinned = nullptr;  // arena_pinned_dots
+};
+
+static std::mutex g_pool_mu;
+static std::vector<Arena*> g_pool;        // arenas no thread owns
+static std::vector<Arena*> g_all;         // every arena (never destroyed)
+static std::atomic<long> g_mallocs{0}, g_hits{0};
+static std::atomic<long long> g_free_bytes{0};
+
+// the calling thread's arena; back to the pool when the thread ends
+struct ArenaOwner {
+  Arena* a = nullptr;
+  int depth = 0;
+  ~ArenaOwner() {
+    if (!a) return;
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    g_pool.push_back(a);
+  }
+};
+static thread_local ArenaOwner t_arena;
+
+ArenaScope::ArenaScope() {
+  if (!t_arena.a) {
+    int dev = 0;
+    TC_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    auto it = std::find_if(g_pool.begin(), g_pool.end(),
+                           [&](Arena* a) { return a->dev == dev; });
+    if (it != g_pool.end()) {
+      t_arena.a = *it;
+      g_pool.erase(it);
+    } else {
+      t_arena.a = new Arena;
+      t_arena.a->dev = dev;
+      g_all.push_back(t_arena.a);
+    }
+  }
+  ++t_arena.depth;
+}
+
+ArenaScope::~ArenaScope() { --t_arena.depth; }
+
+static constexpr size_t kArenaGrain = size_t(2) << 20;  // 2 MiB
+
+// frees cached blocks of `a` (caller holds a->mu), largest first, until at
+// most `keep` bytes stay cached
+static void arena_shrink(Arena* a, size_t keep) {
+  while (a->free_bytes > keep && !a->free.empty()) {
+    auto it = std::prev(a->free.end());
+    (void)hipFree(it->second);
+    a->free_bytes -= it->first;
+    g_free_bytes -= (long long)it->first;
+    a->free.erase(it);
+  }
+}
+
+// cached bytes beyond this share of the device's memory are freed
+static size_t cache_limit() {
+  static size_t lim = [] {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess || tot == 0)
+      return size_t(16) << 30;
+    return tot / 5 * 2;
+  }();
+  return lim;
+}
+
+void* arena_alloc(size_t bytes, Arena** owner, size_t* cap) {
+  Arena* a = t_arena.a;
+  if (!a || t_arena.depth == 0) return nullptr;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev != a->dev) return nullptr;
+  const size_t want = (bytes + kArenaGrain - 1) / kArenaGrain * kArenaGrain;
+  {
+    std::lock_guard<std::mutex> lk(a->mu);
+    // the smallest cached block that fits, if it wastes at most half
+    auto it = a->free.lower_bound(want);
+    if (it != a->free.end() && it->first <= 2 * want) {
+      void* p = it->second;
+      *cap = it->first;
+      a->free_bytes -= it->first;
+      g_free_bytes -= (long long)it->first;
+      a->free.erase(it);
+      *owner = a;
+      ++g_hits;
+      return p;
+    }
+  }
+  void* p = nullptr;
+  if (hipMalloc(&p, want) != hipSuccess) {
+    (void)hipGetLastError();
+    // out of memory: give the cache back and try once more
+    {
+      std::lock_guard<std::mutex> lk(a->mu);
+      arena_shrink(a, 0);
+    }
+    arena_trim();
+    TC_HIP(hipMalloc(&p, want));
+  }
+  ++g_mallocs;
+  *cap = want;
+  *owner = a;
+  return p;
+}
+
+void arena_free(void* p, size_t cap, Arena* owner) {
+  if (owner != t_arena.a) {
+    // another thread's block: its stream work may still be pending, and
+    // hipFree waits for the device
+    (void)hipFree(p);
+    return;
+  }
+  std::lock_guard<std::mutex> lk(owner->mu);
+  owner->free.emplace(cap, p);
+  owner->free_bytes += cap;
+  g_free_bytes += (long long)cap;
+  if ((size_t)std::max<long long>(0, g_free_bytes.load()) > cache_limit())
+    arena_shrink(owner, owner->free_bytes / 2);
+}
+
+double* arena_pinned_dots() {
+  Arena* a = t_arena.a;
+  TC_REQUIRE(a && t_arena.depth > 0, "pinned dots need an arena scope");
+  if (!a->pinned)
+    TC_HIP(hipHostMalloc((void**)&a->pinned, sizeof(double) * 8,
+                         hipHostMallocDefault));
+  return a->pinned;
+}
+
+void arena_stats(long* mallocs, long* hits, double* cached_bytes) {
+  if (mallocs) *mallocs = g_mallocs.load();
+  if (hits) *hits = g_hits.load();
+  if (cached_bytes) *cached_bytes = (double)g_free_bytes.load();
+}
+
+void arena_trim() {
+  std::vector<Arena*> as;
+  {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    as = g_pool;
+  }
+  if (t_arena.a) as.push_back(t_arena.a);
+  for (Arena* a : as) {
+    std::lock_guard<std::mutex> lk(a->mu);
+    arena_shrink(a, 0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// kernel timing
+// ---------------------------------------------------------------------------
 
 // Event pairs recorded around launches of enabled kernel classes; resolved
 // (and released) by ktimer_collect.

@@ -64,6 +64,38 @@ int guarded(F&& f) {
 // device buffers
 // ---------------------------------------------------------------------------
 
+// Device-memory cache of the calling host thread.  Inside an ArenaScope,
+// DevBuf takes its memory from the thread's arena: released buffers go back
+// to the arena's free list (not to hipFree, which synchronises the whole
+// device) and later requests of a similar size reuse them, so a repeated
+// MBIR call allocates nothing after the first.  An arena is owned by one
+// host thread at a time (reuse is ordered by that thread's stream work);
+// when the thread ends, its arena goes back to a process-wide pool for the
+// next thread, with its blocks still cached.  Scopes belong around work
+// that synchronises its streams before the scope ends.
+struct Arena;
+class ArenaScope {
+ public:
+  ArenaScope();
+  ~ArenaScope();
+  ArenaScope(const ArenaScope&) = delete;
+  ArenaScope& operator=(const ArenaScope&) = delete;
+};
+// A cached block of at least `bytes` from the thread's arena when a scope is
+// active (its arena and capacity returned), else nullptr.
+void* arena_alloc(size_t bytes, Arena** owner, size_t* cap);
+// Returns a block to `owner` (its owning thread) or frees it.
+void arena_free(void* p, size_t cap, Arena* owner);
+// Device allocations made by arenas, requests served from a cache, and the
+// bytes cached (free) across all arenas.
+void arena_stats(long* mallocs, long* hits, double* cached_bytes);
+// Frees every cached (free) block of every pooled arena and of the calling
+// thread's arena.
+void arena_trim();
+// Eight doubles of pinned host memory belonging to the calling thread's
+// arena (inside an ArenaScope), kept with the arena for the process.
+double* arena_pinned_dots();
+
 class DevBuf {
  public:
   DevBuf() = default;
@@ -71,31 +103,54 @@ class DevBuf {
   ~DevBuf() { release(); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+  DevBuf(DevBuf&& o) noexcept
+      : p_(o.p_), n_(o.n_), cap_(o.cap_), arena_(o.arena_) {
+    o.p_ = nullptr;
+    o.n_ = o.cap_ = 0;
+    o.arena_ = nullptr;
+  }
   DevBuf& operator=(DevBuf&& o) noexcept {
     if (this != &o) {
       release();
       p_ = o.p_;
       n_ = o.n_;
+      cap_ = o.cap_;
+      arena_ = o.arena_;
       o.p_ = nullptr;
-      o.n_ = 0;
+      o.n_ = o.cap_ = 0;
+      o.arena_ = nullptr;
     }
     return *this;
   }
   void alloc(size_t bytes) {
     release();
     if (bytes == 0) return;
-    TC_HIP(hipMalloc(&p_, bytes));
+    p_ = arena_alloc(bytes, &arena_, &cap_);
+    if (!p_) {
+      TC_HIP(hipMalloc(&p_, bytes));
+      cap_ = bytes;
+    }
     n_ = bytes;
   }
   // grows (never shrinks) the allocation
   void reserve(size_t bytes) {
-    if (bytes > n_) alloc(bytes);
+    if (bytes <= n_) return;
+    if (p_ && bytes <= cap_) {
+      n_ = bytes;
+      return;
+    }
+    alloc(bytes);
   }
   void release() {
-    if (p_) (void)hipFree(p_);
+    if (p_) {
+      if (arena_)
+        arena_free(p_, cap_, arena_);
+      else
+        (void)hipFree(p_);
+    }
     p_ = nullptr;
-    n_ = 0;
+    n_ = cap_ = 0;
+    arena_ = nullptr;
   }
   template <typename T>
   T* as() const { return reinterpret_cast<T*>(p_); }
@@ -104,6 +159,8 @@ class DevBuf {
  private:
   void* p_ = nullptr;
   size_t n_ = 0;
+  size_t cap_ = 0;
+  Arena* arena_ = nullptr;
 };
 
 inline unsigned cdiv(size_t a, size_t b) { return unsigned((a + b - 1) / b); }

@@ -72,8 +72,9 @@ class Psf {
   // nothing, otherwise).  The input is FP16 with a per-slice power-of-two
   // scale on the device: x[s] = g16[s] / gsc[s].  The passes compute in FP32
   // and store their intermediates in FP16 (pass A output scaled by a power
-  // of two fixed by n, pass B output with a per-column scale picked from the
-  // column's maximum), and read the spectrum from an FP16 copy.  The dot
+  // of two fixed by n, pass B output with a power-of-two scale per 32
+  // threads of a column workgroup, from their maximum), and read the
+  // spectrum from an FP16 copy.  The dot
   // partials, T x and the committed step are in the input's true units.
   bool apply_half(const void* g16, const float* gsc, float* out, int nslices,
                   double* dot_partials, hipStream_t stream);
@@ -114,14 +115,15 @@ class Psf {
   size_t work_slices_ = 0;
   DevBuf spec16_;  // FP16 spectrum * 2^spec16_exp_, made on first use
   int spec16_exp_ = 0;
-  DevBuf cinv_;    // per-column inverse scales of pass B's FP16 output
+  DevBuf cinv_;    // inverse scales of pass B's FP16 output (per 32 threads)
   size_t cinv_slices_ = 0;
 };
 
 // Scales of the mixed-precision passes (see Psf::apply_half).
 struct ToeplitzHalf {
   const float* gsc;  // per-slice input scale (device)
-  float* cinv;       // [slice][H] written by pass B, read by pass C
+  float* cinv;       // written by pass B (one per 32 threads of a column
+                     // workgroup), read by pass C
   float wsc;         // pass A output scale: n * 2^16 * wsc <= 2^15
   float cbase;       // 1 / (2^e wsc), e the spectrum copy's exponent
   float pconst;      // 1 / (2^e wsc^2): spectral dot back to input units

@@ -307,6 +307,7 @@ bool toeplitz_apply_ct_half(int M, const void* x16, float* out, void* wt,
                             const ToeplitzHalf& half,
                             const ToeplitzStreams* ts);
 int toeplitz_half_group(int nslices, int H, int rp);
+int toeplitz_ct_scales(int M, int H);
 
 // max |v[i]| over n floats into out[0] (one workgroup)
 __global__ void absmax_kernel(const float* __restrict__ v, long n,
@@ -358,10 +359,10 @@ bool Psf::run_half(const void* g16, const float* gsc, float* out,
     TC_HIP(hipGetLastError());
   }
   reserve(nslices);
-  // per-column scales for two groups in flight (two-stream passes)
+  // pass B's output scales for two groups in flight (two-stream passes)
   const int group = 2 * toeplitz_half_group(nslices, H_, rp_);
   if ((size_t)group > cinv_slices_) {
-    cinv_.alloc(sizeof(float) * (size_t)group * H_);
+    cinv_.alloc(sizeof(float) * (size_t)group * toeplitz_ct_scales(M_, H_));
     cinv_slices_ = group;
   }
   int nb = 0;

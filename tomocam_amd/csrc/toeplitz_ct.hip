@@ -43,6 +43,23 @@ struct Xform<L, E, DIR, Radices<Rs...>> {
 // Plans: forward radices Fwd, inverse radices Inv = reversed Fwd, LPW lines
 // per workgroup in the row passes and LPWC in the column pass.  ZA: pass A
 // skips the zero padding of its input in the first FFT stage.
+template <int L, class Rad>
+struct LineElems;
+template <int L, int... Rs>
+struct LineElems<L, Radices<Rs...>> {
+  static constexpr int value = ct::line_elems<L, Rs...>();
+};
+
+// Stride between the LDS lines of a workgroup with `lines` lines: at least
+// `elems`, and congruent to ceil(32 / lines) modulo 32, so that the row
+// passes' Hermitian loops (lane q on line q mod lines, element q / lines)
+// spread a wave over distinct banks.
+constexpr int line_stride(int elems, int lines) {
+  if (lines <= 1) return elems;
+  const int target = (32 + lines - 1) / lines % 32;
+  return elems + (((target - elems) % 32) + 32) % 32;
+}
+
 template <int L_, int E_, int LPW_, class Fwd_, class Inv_, int LPWC_ = LPW_,
           bool ZA_ = true>
 struct TPlan {
@@ -50,10 +67,36 @@ struct TPlan {
   static constexpr int NT = LPW_ * (L_ / E_);
   static constexpr int LPWC = LPWC_;
   static constexpr int NTC = LPWC_ * (L_ / E_);
+  static constexpr int NTCL = (NTC + 63) / 64 * 64;  // column-pass block
   static constexpr bool ZA = ZA_;
   using Fwd = Fwd_;
   using Inv = Inv_;
+  // LDS elements per line (padded exchanges) and line strides of the row
+  // passes (LS) and the column pass (LSC)
+  static constexpr int LE = LineElems<L_, Fwd_>::value > LineElems<L_, Inv_>::value
+                                ? LineElems<L_, Fwd_>::value
+                                : LineElems<L_, Inv_>::value;
+  static constexpr int LS = line_stride(LE, LPW_);
+  static constexpr int LSC = line_stride(LE, LPWC_);
+  // FP16 column pass: one scale per 32 consecutive threads of a workgroup
+  // (SG per workgroup); scales per slice for H columns
+  static constexpr int SG = (LPWC_ * (L_ / E_) + 31) / 32;
+  static constexpr int scales_per_slice(int H) {
+    return (H + LPWC_ - 1) / LPWC_ * SG;
+  }
 };
+
+// The scale group of element `row` of column k1 in the FP16 column pass's
+// output: the 32-thread group of the workgroup thread that stored it
+// (thread (k1 mod LPWC) T + row mod T; the inverse's last stage stores
+// rows t + r L/R1, and T = L/R1 when E = R1).
+template <class P>
+__device__ __forceinline__ int col_scale_index(int s, int k1, int row, int H) {
+  constexpr int T = P::T, LPWC = P::LPWC;
+  const int kg = k1 / LPWC;
+  const int tid = (k1 - kg * LPWC) * T + row % T;
+  return (s * ((H + LPWC - 1) / LPWC) + kg) * P::SG + (tid >> 5);
+}
 
 using Plan750 = TPlan<750, 30, 10, Radices<10, 15, 5>, Radices<5, 15, 10>>;
 using Plan1458 = TPlan<1458, 18, 3, Radices<18, 9, 9>, Radices<9, 9, 18>>;
@@ -184,7 +227,7 @@ ct_rows_fwd(const void* __restrict__ xv, void* __restrict__ wtv,
         }
       }
   }
-  float2* ln = sm + line * L;
+  float2* ln = sm + line * P::LS;
   F::template run<P::ZA>(v, ln, t, true, tw);
 #pragma unroll
   for (int i = 0; i < E / RS; ++i)
@@ -197,8 +240,8 @@ ct_rows_fwd(const void* __restrict__ xv, void* __restrict__ wtv,
     const int l = q - k * LPW;
     const int rr = r0 + 2 * l;
     if (rr < rp) {
-      const float2 zk = sm[l * L + k];
-      const float2 zm = cconj(sm[l * L + (k == 0 ? 0 : L - k)]);
+      const float2 zk = sm[l * P::LS + k];
+      const float2 zm = cconj(sm[l * P::LS + (k == 0 ? 0 : L - k)]);
       const float2 sp = cadd(zk, zm), df = csub(zk, zm);
       if constexpr (H16) {
         // |value| <= n * 2^16 * wsc <= 2^15: no overflow
@@ -221,14 +264,13 @@ ct_rows_fwd(const void* __restrict__ xv, void* __restrict__ wtv,
 // ---------------------------------------------------------------------------
 
 template <class P, bool H16>
-__global__ void __launch_bounds__(P::NTC)
+__global__ void __launch_bounds__(P::NTCL)
 ct_cols(void* __restrict__ wtv, const void* __restrict__ specv,
         const float2* __restrict__ tw, int n, int H, int rp, int nbps,
         double* partials, ToeplitzHalf ha) {
   extern __shared__ float2 sm[];
   __shared__ double red[16];
   constexpr int L = P::L, E = P::E, T = P::T, LPW = P::LPWC;
-  __shared__ unsigned cmax[H16 ? LPW : 1];
   using F = Xform<L, E, -1, typename P::Fwd>;
   using I = Xform<L, E, 1, typename P::Inv>;
   constexpr int R1 = F::first, RS = F::last;
@@ -242,11 +284,10 @@ ct_cols(void* __restrict__ wtv, const void* __restrict__ specv,
   const int kg = b / nslices;
   const int s = b - kg * nslices;
   const int k1 = kg * LPW + line;
-  const bool active = k1 < H;
+  // threads past the last line (the block is rounded up to whole waves)
+  // are inactive
+  const bool active = line < LPW && k1 < H;
   const int kc = active ? k1 : 0;
-  if constexpr (H16) {
-    if (threadIdx.x < LPW) cmax[threadIdx.x] = 0u;
-  }
   float2 v[E];
 #pragma unroll
   for (int i = 0; i < E / R1; ++i)
@@ -279,7 +320,7 @@ ct_cols(void* __restrict__ wtv, const void* __restrict__ specv,
           hs[i * RS + r] = reinterpret_cast<const float*>(specv)[o];
       }
   }
-  float2* ln = sm + line * L;
+  float2* ln = sm + line * P::LSC;
   F::template run<true>(v, ln, t, active, tw);
   double acc = 0.0;
   if (active) {
@@ -299,8 +340,11 @@ ct_cols(void* __restrict__ wtv, const void* __restrict__ specv,
   // the inverse's first stage needs no exchange (same element set)
   I::run(v, ln, t, active, tw);
   if constexpr (H16) {
-    // per-column power-of-two scale: the column's largest kept value maps
-    // below 2^14
+    // Power-of-two scale per 32 consecutive threads (a half wave): their
+    // largest kept value maps below 2^14.  A shuffle reduction within the
+    // half wave (the workgroup is a whole number of waves, so every lane
+    // exists; inactive ones contribute 0); no LDS and no barrier.  Thread
+    // t stores rows t + i T + r L/R1, i.e. the rows congruent to t mod T.
     float m = 0.f;
     if (active) {
 #pragma unroll
@@ -309,16 +353,19 @@ ct_cols(void* __restrict__ wtv, const void* __restrict__ specv,
         for (int r = 0; r < R1 / 2; ++r) {
           const int idx = t + i * T + r * (L / R1);
           if (idx < n)
-            m = fmaxf(m, fmaxf(fabsf(v[i * R1 + r].x), fabsf(v[i * R1 + r].y)));
+            m = fmaxf(m, fmaxf(fabsf(v[i * R1 + r].x),
+                               fabsf(v[i * R1 + r].y)));
         }
-      atomicMax(&cmax[line], __float_as_uint(m));
     }
-    __syncthreads();
+#pragma unroll
+    for (int off = 16; off > 0; off >>= 1)
+      m = fmaxf(m, __shfl_xor(m, off, 32));
+    // m < 2^ex (frexp's exponent); all-zero and tiny groups keep ex >= -100
+    int ex = 14;
+    if (m > 0.f && m <= 3.0e38f)
+      ex = max((int)((__float_as_uint(m) >> 23) & 0xff) - 126, -100);
+    const float kap = __uint_as_float((unsigned)(127 + 14 - ex) << 23);
     if (active) {
-      const float mm = __uint_as_float(cmax[line]);
-      int ex = 14;
-      if (mm > 0.f && mm <= 3.0e38f) (void)frexpf(mm, &ex);
-      const float kap = ldexpf(1.f, 14 - ex);
 #pragma unroll
       for (int i = 0; i < E / R1; ++i)
 #pragma unroll
@@ -328,8 +375,11 @@ ct_cols(void* __restrict__ wtv, const void* __restrict__ specv,
             reinterpret_cast<unsigned*>(wtv)[wt_at(s, kc, idx, H, rp)] =
                 pack_half2(kap * v[i * R1 + r].x, kap * v[i * R1 + r].y);
         }
-      if (t == 0) ha.cinv[(long)s * H + k1] = ldexpf(ha.cbase, ex - 14);
     }
+    if ((threadIdx.x & 31) == 0 && (int)threadIdx.x < P::NTC)
+      ha.cinv[((long)s * ((H + LPW - 1) / LPW) + kg) * P::SG +
+              (threadIdx.x >> 5)] =
+          ha.cbase * __uint_as_float((unsigned)(127 + ex - 14) << 23);
   } else if (active) {
 #pragma unroll
     for (int i = 0; i < E / R1; ++i)
@@ -375,14 +425,14 @@ ct_rows_inv(const void* __restrict__ wtv, const float* bvec, float* out,
   constexpr int HC = L / 2 + 1;
   constexpr int NQ = (LPW * HC + P::NT - 1) / P::NT;
   float4 w4[NQ];
-  float cs[H16 ? NQ : 1];
+  float cs[H16 ? NQ : 1], cs1[H16 ? NQ : 1];
 #pragma unroll
   for (int j = 0; j < NQ; ++j) {
     const int q = threadIdx.x + j * P::NT;
     const int k = q / LPW;
     const int rr = r0 + 2 * (q - k * LPW);
     w4[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if constexpr (H16) cs[j] = 0.f;
+    if constexpr (H16) cs[j] = cs1[j] = 0.f;
     if (q < LPW * H && rr < rp) {
       const long o = wt_at(s, k, rr, H, rp);
       if constexpr (H16) {
@@ -390,7 +440,11 @@ ct_rows_inv(const void* __restrict__ wtv, const float* bvec, float* out,
             reinterpret_cast<const unsigned*>(wtv) + o);
         const float2 a = unpack_half2(u.x), c = unpack_half2(u.y);
         w4[j] = make_float4(a.x, a.y, c.x, c.y);
-        cs[j] = ha.cinv[(long)s * H + k];
+        // rows rr and rr + 1 may lie in different scale groups (odd T)
+        const int i0 = col_scale_index<P>(s, k, rr, H);
+        const int i1 = col_scale_index<P>(s, k, rr + 1, H);
+        cs[j] = ha.cinv[i0];
+        cs1[j] = i1 == i0 ? cs[j] : ha.cinv[i1];
       } else {
         w4[j] = *reinterpret_cast<const float4*>(
             reinterpret_cast<const float2*>(wtv) + o);
@@ -407,20 +461,20 @@ ct_rows_inv(const void* __restrict__ wtv, const float* bvec, float* out,
       if constexpr (H16) {
         w.x *= cs[j];
         w.y *= cs[j];
-        w.z *= cs[j];
-        w.w *= cs[j];
+        w.z *= cs1[j];
+        w.w *= cs1[j];
       }
       if (k == 0 || 2 * k == L) {
         w.y = 0.f;
         w.w = 0.f;
       }
-      sm[l * L + k] = make_float2(w.x - w.w, w.y + w.z);
+      sm[l * P::LS + k] = make_float2(w.x - w.w, w.y + w.z);
       if (k > 0 && L - k >= H)
-        sm[l * L + L - k] = make_float2(w.x + w.w, w.z - w.y);
+        sm[l * P::LS + L - k] = make_float2(w.x + w.w, w.z - w.y);
     }
   }
   __syncthreads();
-  float2* ln = sm + line * L;
+  float2* ln = sm + line * P::LS;
   float2 v[E];
 #pragma unroll
   for (int i = 0; i < E / R1; ++i)
@@ -575,7 +629,7 @@ static int toeplitz_group_slices(int nslices, int H, int rp, int elem_bytes) {
   return toeplitz_rule_group(nslices, H, rp, elem_bytes);
 }
 
-// Slices per group for the FP16-storage passes (sizes the per-column scale
+// Slices per group for the FP16-storage passes (sizes the pass-B scale
 // buffer).
 int toeplitz_half_group(int nslices, int H, int rp) {
   return toeplitz_group_slices(nslices, H, rp, 4);
@@ -588,8 +642,8 @@ static void run_ct(const void* x, const float* b, float* out, void* wt,
                    double img_b, double spec_b, const Psf::StepCommit* sc,
                    const ToeplitzHalf* half, const ToeplitzStreams* ts) {
   const float2* tw = fft_length(P::L).plan().tw;
-  const size_t smem = sizeof(float2) * P::LPW * P::L;
-  const size_t smemc = sizeof(float2) * P::LPWC * P::L;
+  const size_t smem = sizeof(float2) * P::LPW * P::LS;
+  const size_t smemc = sizeof(float2) * P::LPWC * P::LSC;
   const int npairs = (n + 1) / 2;
   const int nbr = (int)cdiv(npairs, P::LPW);  // row-pass blocks per slice
   const int nbc = (int)cdiv(H, P::LPWC);      // column-pass blocks per slice
@@ -625,7 +679,7 @@ static void run_ct(const void* x, const float* b, float* out, void* wt,
     if (half) {
       ha = *half;
       ha.gsc = half->gsc + s0;
-      if (odd) ha.cinv = half->cinv + (long)group * H;
+      if (odd) ha.cinv = half->cinv + (long)group * P::scales_per_slice(H);
     }
     const void* xg = static_cast<const char*>(x) + xel * o;
     const float* bg = b ? b + o : nullptr;
@@ -649,7 +703,7 @@ static void run_ct(const void* x, const float* b, float* out, void* wt,
       auto k = ct_cols<P, H16>;
       allow_lds((const void*)k, smemc);
       ktimer_begin(KC_TOEPLITZ_COLS, stream);
-      hipLaunchKernelGGL(k, dim3(nbc * ns), dim3(P::NTC), smemc, stream, wtg,
+      hipLaunchKernelGGL(k, dim3(nbc * ns), dim3(P::NTCL), smemc, stream, wtg,
                          spec, tw, n, H, rp, nbc,
                          partials ? partials + (long)s0 * nbc : nullptr, ha);
       TC_HIP(hipGetLastError());
@@ -706,6 +760,22 @@ int toeplitz_ct_lpw(int M) {
     case 2304: return Plan2304::LPWC;
     case 768: return Plan768::LPWC;
     case 5832: return Plan5832::LPWC;
+    default: return 0;
+  }
+}
+
+// FP16 column-pass scales per slice (0: no compile-time plan for M)
+int toeplitz_ct_scales(int M, int H) {
+  switch (M) {
+    case 750: return Plan750::scales_per_slice(H);
+    case 1458: return Plan1458::scales_per_slice(H);
+    case 2250: return Plan2250::scales_per_slice(H);
+    case 2916: return Plan2916::scales_per_slice(H);
+    case 4500: return Plan4500::scales_per_slice(H);
+    case 4608: return Plan4608::scales_per_slice(H);
+    case 2304: return Plan2304::scales_per_slice(H);
+    case 768: return Plan768::scales_per_slice(H);
+    case 5832: return Plan5832::scales_per_slice(H);
     default: return 0;
   }
 }

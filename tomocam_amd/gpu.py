@@ -249,6 +249,22 @@ def ktimer_collect(kernel_class: int) -> Tuple[float, int, float]:
     return ms.value, cnt.value, nb.value
 
 
+def alloc_stats() -> Tuple[int, int, float]:
+    """(device allocations made by the library's caches, requests served
+    from a cache, bytes currently cached)."""
+    m = C.c_long()
+    h = C.c_long()
+    b = C.c_double()
+    _lib.call("tc_alloc_stats", C.byref(m), C.byref(h), C.byref(b))
+    return m.value, h.value, b.value
+
+
+def release_cached() -> None:
+    """Frees the library's cached device memory (this thread's and idle
+    caches)."""
+    _lib.call("tc_release_cached")
+
+
 def _angles(angles) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(angles, dtype=np.float32).ravel())
 
