@@ -1,19 +1,26 @@
 #!/usr/bin/env python3
 """MBIR throughput benchmark (slices/s) on one MI355X.
 
-Workload (default): plain FP32 Toeplitz-gradient MBIR on 32 slices of
-1024 x 1024, 720 angles uniform in [0, pi) x 1024 rays, centre 512, seeded
-30-random-ellipse phantoms projected with the library's NUFFT projector plus
-1% Gaussian noise, 100 iterations, qGGMRF p = 1.2, q = 2, c = 0.001,
-sigma = 0.1.  One step = one full mbir2 call (normalise, FBP initial guess,
-padding, P^T y, PSF, step size, 100 iterations, crop) on a sinogram already
-resident in HBM.
+Headline workload (default): Toeplitz-gradient MBIR on 64 slices of
+2048 x 2048, 1501 angles uniform in [0, pi) x 2048 rays, centre 1024,
+seeded 30-random-ellipse phantoms projected with the library's NUFFT
+projector plus 0.5 % Gaussian noise, 100 iterations, qGGMRF p = 1.2, q = 2,
+c = 0.001, sigma = 100.  One step = one full mbir2 call (normalise, FBP
+initial guess, padding, P^T y, PSF, step size, 100 iterations, crop) on a
+sinogram already resident in HBM.  `value` is the mixed-precision form's
+slices/s; plain FP32 is timed beside it on the same input.
 
-Prints one JSON line with the throughput, the roofline of the dominant
-kernel (HIP-event timed inside the timed region), a CPU baseline (the FP32
-CPU restatement of the same algorithm on a bounded sample) and the
-mixed-precision form against plain FP32 on 64 x 2048^2 (one timed call each,
-sigma = 100).
+The JSON line also carries:
+  - `roofline`: the dominant kernel class of the timed (mixed) run, its
+    algorithmic bytes over its HIP-event time on the launch stream;
+  - `accuracy`: both forms against the FP64 oracle on one 2048^2 slice of
+    the workload's geometry at 100 iterations (a committed fixture), and
+    mixed against FP32 on the whole timed output;
+  - `cpu_baseline`: the FP32 CPU restatement of the same algorithm on a
+    bounded sample (set-up once, one warm-up iteration, the median of three
+    timed runs), extrapolated per slice-iteration;
+  - `extra`: the other workload shapes (32 x 1024^2 FP32, the 200-stack
+    uneven batch, the autograd projector step).
 """
 
 import argparse
@@ -29,6 +36,17 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 WORKLOADS = {
+    # the headline: 100-iteration MBIR at 2048^2, mixed precision against
+    # plain FP32 (sigma = 100, the reference's default smoothness 0.01: the
+    # iteration converges, so errors against FP64 are meaningful)
+    "mbir-64x2048": dict(
+        slices=64, npix=2048, nproj=1501, iterations=100, sigma=100.0, p=1.2,
+        noise=0.005, seed_group=3, lipschitz_iters=20,
+        label="Toeplitz MBIR, mixed precision (FP16 gradient and PSF "
+              "spectrum storage, FP32 FFTs, intermediates and state, FP64 "
+              "line search, L from 20 power-iteration steps) against plain "
+              "FP32: 64 slices 2048x2048, 1501 angles x 2048 rays, "
+              "100 iterations, qGGMRF sigma=100 p=1.2"),
     "plain-fp32-32x1024": dict(
         slices=32, npix=1024, nproj=720, iterations=100, sigma=0.1, p=1.2,
         noise=0.01, seed_group=1,
@@ -45,10 +63,6 @@ WORKLOADS = {
         label="uneven batching: 200 seeded stacks, image size in "
               "{256,512,768,1024,1536}, angles U{180..1500}, rays = size, "
               "1-16 slices each, 50 iterations, qGGMRF sigma=0.1 p=1.2"),
-    "mixed-size-64x2048": dict(
-        slices=64, npix=2048, nproj=1501, iterations=100, sigma=0.1, p=1.2,
-        noise=0.005, seed_group=3,
-        label="64 slices 2048x2048, 1501 angles, 100 iterations (plain FP32)"),
     # one forward + backward of the differentiable loss
     # 0.5 ||P x - y||^2 + beta qGGMRF(x) (projector with the hand-written
     # adjoint backward, free-boundary prior with its exact gradient kernel)
@@ -57,55 +71,55 @@ WORKLOADS = {
         sigma=0.1, p=1.2, beta=1e-3, noise=0.01, seed_group=2,
         label="differentiation: forward + backward of 0.5||Px-y||^2 + "
               "1e-3 qGGMRF(x) on 8 slices 512x512, 360 angles x 512 rays"),
-    # the mixed-precision comparison: sigma = 100 (the reference's default
-    # smoothness 0.01), where the iteration converges and relative errors
-    # between precisions are meaningful
-    "mixed-prec-64x2048": dict(
-        slices=64, npix=2048, nproj=1501, iterations=100, sigma=100.0, p=1.2,
-        noise=0.005, seed_group=3, precision="mixed",
-        label="mixed precision (FP16 gradient and Toeplitz intermediates, "
-              "FP32 FFTs and state, FP64 dots, FP32 refresh every 5): 64 "
-              "slices 2048x2048, 1501 angles x 2048 rays, 100 iterations, "
-              "qGGMRF sigma=100 p=1.2"),
 }
+HEADLINE = "mbir-64x2048"
 
 PEAK_HBM_GBS = 8000.0
 
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
-# counter runs (calibrated on a copy kernel) for the default workload
-TRAFFIC_FILE = os.path.join(ROOT, "profiles",
-                            "mbir_fp32_32x1024_hbm_traffic.json")
+# counter runs (calibrated on a copy kernel), per workload
+TRAFFIC_FILES = {
+    "plain-fp32-32x1024": "mbir_fp32_32x1024_hbm_traffic.json",
+    "mbir-64x2048": "mbir_mixed_64x2048_hbm_traffic.json",
+}
 
 
 PEAK_VALU_TFLOPS = 78.6  # FP32 vector FMA without packed math (2 flop/lane/clk)
 
 
-def valu_roofline(kd, n, L, half=False):
+def valu_roofline(kd, kernel, n, L, wt_bytes=8, spec_bytes=4):
     """The column pass against the FP32 vector-issue peak.  Its time goes to
     FFT arithmetic (VALU bound), so next to the HBM roofline it is also
     rated by algorithmic flops: 5 L log2(L) per length-L complex transform,
     two per column (forward and inverse), plus the 2L of the real spectrum
     multiply; H = L/2 + 1 columns per slice.  The slices per launch follow
-    from the launch's algorithmic bytes (ns * 2 * 8nH + 4HL in FP32,
-    ns * 2 * 4nH + 2HL with FP16 intermediates)."""
+    from the launch's algorithmic bytes (ns * 2 * wt_bytes * n * H +
+    spec_bytes * H * L)."""
     import math
     H = L // 2 + 1
-    if half:
-        ns = (kd["bytes_per_launch"] - 2.0 * H * L) / (8.0 * n * H)
-    else:
-        ns = (kd["bytes_per_launch"] - 4.0 * H * L) / (16.0 * n * H)
+    ns = (kd["bytes_per_launch"] - spec_bytes * H * L) / (2.0 * wt_bytes * n * H)
     flops = ns * H * (2 * 5.0 * L * math.log2(L) + 2.0 * L)
     ach = flops / (kd["avg_ms"] * 1e-3) / 1e12
-    return {"kernel": "toeplitz_cols", "bound": "valu", "achieved": ach,
+    return {"kernel": kernel, "bound": "valu", "achieved": ach,
             "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
             "frac": ach / PEAK_VALU_TFLOPS, "flops_per_launch": flops,
             "slices_per_launch": ns, "fft_length": L}
 
 
+def fft_length(n):
+    import ctypes as C
+    from tomocam_amd import _lib
+    L = C.c_int()
+    _lib.call("tc_toeplitz_fft_length", int(n), C.byref(L))
+    return L.value
+
+
 def measured_traffic(workload, kernel_class):
-    if workload != "plain-fp32-32x1024" or not os.path.exists(TRAFFIC_FILE):
+    name = TRAFFIC_FILES.get(workload)
+    path = os.path.join(ROOT, "profiles", name) if name else None
+    if not path or not os.path.exists(path):
         return None
-    with open(TRAFFIC_FILE) as f:
+    with open(path) as f:
         t = json.load(f)
     c = t["classes"].get(kernel_class)
     return None if c is None else c["traffic_bytes"]
@@ -182,55 +196,184 @@ def make_uneven(torch, gpu, phantom, w):
     return stacks
 
 
-def cpu_baseline(sino_host, ang, w, sample_slices, sample_iters):
+def cpu_baseline(sino_host, ang, w, total_slices, sample_slices,
+                 sample_iters):
+    """The FP32 CPU restatement on `sample_slices` slices: set-up once, one
+    warm-up iteration, the median of three runs of `sample_iters`
+    iterations, extrapolated to the workload (per slice-iteration; the PSF
+    once per stack)."""
     from oracle import cpu_port
     r = cpu_port.sample_rate(sino_host, ang, w["npix"] / 2.0, w["iterations"],
-                             w["sigma"], sample_slices, sample_iters)
+                             w["sigma"], sample_slices, sample_iters,
+                             total_slices=total_slices)
     return {
         "value": r["slices_per_s"],
         "unit": "slices/s",
         "cores": r["threads"],
         "kind": "port",
         "sample": (f"FP32 CPU restatement of mbir2 (4 Toeplitz products per "
-                   f"attempt, scipy.fft + C/OpenMP): full setup and "
-                   f"{sample_iters} iterations on {r['sample_slices']} of the "
-                   f"{w['slices']} slices ({r['setup_s']:.1f} s setup, "
-                   f"{r['iter_s']:.1f} s iterations), extrapolated linearly to "
+                   f"attempt, scipy.fft + C/OpenMP) on {r['sample_slices']} "
+                   f"of the {total_slices} slices: set-up once "
+                   f"({r['setup_s']:.1f} s per sample, PSF {r['psf_s']:.1f} s "
+                   f"per stack), 1 warm-up iteration, median of 3 runs of "
+                   f"{sample_iters} iterations ({r['iter_s']:.2f} s per "
+                   f"iteration); extrapolated linearly to "
                    f"{w['iterations']} iterations per slice"),
     }
 
 
-def mixed_compare(torch, gpu, phantom, warmup):
-    """Mixed precision against plain FP32 on the 64 x 2048^2 workload: one
-    timed call of each (after `warmup` short calls priming the plans), their
-    slices/s and the relative L2 difference of the reconstructions."""
-    w = dict(WORKLOADS["mixed-prec-64x2048"])
-    sino, ang = make_sinogram(torch, gpu, phantom, w)
-    center = w["npix"] / 2.0
-    res, recs = {}, {}
-    for prec in ("fp32", "mixed"):
-        for _ in range(warmup):
-            gpu.mbir(sino, ang, center, 2, w["sigma"], precision=prec)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        rec, info = gpu.mbir(sino, ang, center, w["iterations"], w["sigma"],
-                             precision=prec)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        res[prec] = (w["slices"] / dt, info)
-        recs[prec] = rec
-        log(f"{prec}: {w['slices'] / dt:.3f} slices/s ({dt:.2f} s)")
-    d = (recs["mixed"].double() - recs["fp32"].double()).norm()
-    rel = float(d / recs["fp32"].double().norm())
+def timed_steps(torch, gpu, step, steps, warmup, classes, what):
+    """`warmup` untimed calls, then `steps` timed ones with the kernel
+    classes' HIP-event timers on; returns (seconds, last result, kernels)."""
+    res = None
+    def brief(r):
+        return r[1] if isinstance(r[1], dict) else f"{len(r[1])} results"
+
+    for i in range(warmup):
+        res = step()
+        log(f"{what}: warmup {i} done: {brief(res)}")
+    for k in classes:
+        gpu.ktimer_collect(k)  # discard earlier records
+        gpu.ktimer_enable(k, True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res = step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    kernels = {}
+    for k in classes:
+        ms, cnt, nbytes = gpu.ktimer_collect(k)
+        gpu.ktimer_enable(k, False)
+        if cnt:
+            kernels[gpu.KERNEL_CLASSES[k]] = dict(
+                launches=cnt, total_ms=ms, avg_ms=ms / cnt,
+                bytes_per_launch=nbytes / cnt,
+                gbs=nbytes / (ms * 1e-3) / 1e9)
+    log(f"{what}: {steps} timed steps in {elapsed:.3f} s; info {brief(res)}")
+    return elapsed, res, kernels
+
+
+def roofline_of(kernels, workload):
+    if not kernels:
+        return None
+    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+    kd = kernels[dom]
     return {
-        "workload": w["label"],
-        "fp32_slices_per_s": res["fp32"][0],
-        "mixed_slices_per_s": res["mixed"][0],
-        "speedup": res["mixed"][0] / res["fp32"][0],
-        "relL2_mixed_vs_fp32": rel,
-        "saturated_slice_attempts": int(res["mixed"][1]["saturated"]),
-        "accepted_iterations": int(res["mixed"][1]["iterations"]),
+        "kernel": dom,
+        "bound": "hbm",
+        "achieved": kd["gbs"],
+        "peak": PEAK_HBM_GBS,
+        "unit": "GB/s",
+        "frac": kd["gbs"] / PEAK_HBM_GBS,
+        # HBM bytes per launch (rocprofv3 counters), or None
+        "traffic": measured_traffic(workload, dom),
+        "bytes_per_launch": kd["bytes_per_launch"],
+        "avg_launch_ms": kd["avg_ms"],
     }
+
+
+def accuracy_vs_fp64(torch, gpu, phantom, w):
+    """Both forms on one 2048^2 slice of the workload's geometry against the
+    FP64 oracle's 100-iteration result (tests/golden/
+    mbir2_2048_slice_100it.npz, made by tests/golden/make_mbir2_golden.py
+    from the same phantom and noise seeds).  The input here is projected by
+    the library's FP32 projector, not the oracle's; its checksum agrees
+    with the fixture's to about 1e-6, far below the 1e-4 bound."""
+    path = os.path.join(ROOT, "tests", "golden", "mbir2_2048_slice_100it.npz")
+    if not os.path.exists(path):
+        return None
+    gold = np.load(path)
+    N, P, seed, frac, nseed = 2048, 1501, 3000, 0.005, 3007
+    ang = phantom.angles(P)
+    img = torch.from_numpy(phantom.random_ellipses(N, seed)[None]).cuda()
+    sino = gpu.project(img, ang).cpu().numpy().astype(np.float64)
+    sino = sino + np.random.default_rng(nseed).normal(0, frac * sino.max(),
+                                                      sino.shape)
+    chk = np.array([sino.sum(), np.linalg.norm(sino)])
+    out = {"case": "one 2048^2 slice, 1501 angles, 100 iterations, "
+                   "sigma=100, against the FP64 oracle (sampled 64 x 64 "
+                   "grid + whole-image norm)",
+           "input_checksum_rel_diff": float(np.max(
+               np.abs(chk - gold["sino_checksum"]) /
+               np.abs(gold["sino_checksum"])))}
+    st = torch.tensor(sino, dtype=torch.float32).cuda()
+    k = int(gold["stride"])
+    for prec in ("fp32", "mixed"):
+        kw = dict(lipschitz_iters=w["lipschitz_iters"]) if prec == "mixed" \
+            else {}
+        rec, info = gpu.mbir(st, ang, N / 2, int(gold["iterations"]),
+                             w["sigma"], precision=prec, **kw)
+        im = rec.cpu().numpy()[0].astype(np.float64)
+        samp = im[::k, ::k]
+        d = samp - gold["sampled"]
+        out[prec] = {
+            "relL2_sampled": float(np.linalg.norm(d) /
+                                   np.linalg.norm(gold["sampled"])),
+            "rel_norm_diff": float(abs(np.linalg.norm(im) - gold["l2"]) /
+                                   gold["l2"]),
+            "step": info["step"],
+        }
+    out["reference_step"] = float(gold["step"])
+    out["within_1e-4"] = all(out[p]["relL2_sampled"] <= 1e-4
+                             for p in ("fp32", "mixed"))
+    return out
+
+
+def extra_workloads(torch, gpu, phantom):
+    """The other shapes, one line each: 32 x 1024^2 plain FP32 (3 timed
+    calls), the 200-stack uneven batch (1 timed call), the autograd step
+    (3 timed calls)."""
+    out = {}
+    classes = sorted(gpu.KERNEL_CLASSES)
+    w = dict(WORKLOADS["plain-fp32-32x1024"])
+    sino, ang = make_sinogram(torch, gpu, phantom, w)
+    el, res, ker = timed_steps(
+        torch, gpu, lambda: gpu.mbir(sino, ang, w["npix"] / 2.0,
+                                     w["iterations"], w["sigma"]),
+        3, 1, classes, "32x1024")
+    out["plain-fp32-32x1024"] = {
+        "workload": w["label"], "slices_per_s": w["slices"] * 3 / el,
+        "ms_per_call": el / 3 * 1e3,
+        "roofline": roofline_of(ker, "plain-fp32-32x1024"),
+        "accepted_iterations": int(res[1]["iterations"])}
+    del sino
+    w = dict(WORKLOADS["uneven-200"])
+    stacks = make_uneven(torch, gpu, phantom, w)
+    nsl = sum(int(st[0].shape[0]) for st in stacks)
+    el, res, ker = timed_steps(
+        torch, gpu, lambda: gpu.mbir_batch(stacks, w["iterations"],
+                                           w["sigma"]),
+        1, 1, [], "uneven-200")
+    recs, infos = res
+    out["uneven-200"] = {
+        "workload": w["label"], "stacks": len(stacks), "slices": nsl,
+        "slices_per_s": nsl / el, "s_per_call": el,
+        "all_finite": bool(all(bool(torch.isfinite(r).all()) for r in recs)),
+        "accepted_iterations_min": int(min(i["iterations"] for i in infos))}
+    del stacks, recs
+    from tomocam_amd import autograd as AG
+    w = dict(WORKLOADS["autograd-8x512"])
+    img = torch.from_numpy(phantom.phantom_stack(
+        w["npix"], w["slices"], "ellipses", w["seed_group"])).cuda()
+    ang = phantom.angles(w["nproj"])
+    sino = gpu.project(img, ang)
+    x = torch.zeros_like(img, requires_grad=True)
+
+    def step():
+        x.grad = None
+        loss = AG.mbir_loss(x, sino, ang, beta=w["beta"], sigma=w["sigma"],
+                            p=w["p"])
+        loss.backward()
+        torch.cuda.synchronize()
+        return None, {"loss": float(loss.detach())}
+
+    el, res, _ = timed_steps(torch, gpu, step, 3, 1, [], "autograd-8x512")
+    out["autograd-8x512"] = {"workload": w["label"],
+                             "slices_per_s": w["slices"] * 3 / el,
+                             "ms_per_step": el / 3 * 1e3}
+    torch.cuda.empty_cache()
+    return out
 
 
 def main():
@@ -238,14 +381,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="plain-fp32-32x1024",
-                    choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-sample-slices", type=int, default=8)
-    ap.add_argument("--cpu-sample-iters", type=int, default=12)
+    ap.add_argument("--workload", default=HEADLINE, choices=sorted(WORKLOADS))
+    ap.add_argument("--fp32-steps", type=int, default=5,
+                    help="timed calls of the plain FP32 comparison "
+                         "(headline workload; at most --steps)")
+    ap.add_argument("--cpu-sample-slices", type=int, default=2)
+    ap.add_argument("--cpu-sample-iters", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-mixed", action="store_true",
-                    help="skip the mixed-precision vs FP32 comparison on the "
-                         "64 x 2048^2 workload")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the other workload shapes")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the FP64 fixture comparison")
     ap.add_argument("--no-ktimer", action="store_true",
                     help="do not bracket kernels with HIP events")
     ap.add_argument("--iterations", type=int, default=None,
@@ -278,16 +424,17 @@ def main():
         torch.cuda.synchronize()
         log("copy calibration: 1 GiB read + 1 GiB written")
         del a, c
-    uneven = w.get("kind") == "uneven"
-    autograd = w.get("kind") == "autograd"
-    if autograd:
+    classes = [] if args.no_ktimer else sorted(gpu.KERNEL_CLASSES)
+    headline = args.workload == HEADLINE
+    kind = w.get("kind", "ellipses")
+    extra = {}
+    if kind == "autograd":
         from tomocam_amd import autograd as AG
         img = torch.from_numpy(phantom.phantom_stack(
             w["npix"], w["slices"], "ellipses", w["seed_group"])).cuda()
         ang = phantom.angles(w["nproj"])
         sino = gpu.project(img, ang)
         x = torch.zeros_like(img, requires_grad=True)
-        center = w["npix"] / 2.0
 
         def step():
             x.grad = None
@@ -295,8 +442,8 @@ def main():
                                 sigma=w["sigma"], p=w["p"])
             loss.backward()
             torch.cuda.synchronize()
-            return None, {"loss": float(loss)}
-    elif uneven:
+            return None, {"loss": float(loss.detach())}
+    elif kind == "uneven":
         stacks = make_uneven(torch, gpu, phantom, w)
         w["slices"] = sum(int(st[0].shape[0]) for st in stacks)
         log(f"{len(stacks)} stacks, {w['slices']} slices")
@@ -307,83 +454,73 @@ def main():
     else:
         sino, ang = make_sinogram(torch, gpu, phantom, w)
         center = w["npix"] / 2.0
-
-        prec = w.get("precision", "fp32")
+        prec = "mixed" if headline else "fp32"
+        kw = dict(lipschitz_iters=w["lipschitz_iters"]) if headline else {}
 
         def step():
             return gpu.mbir(sino, ang, center, w["iterations"], w["sigma"],
-                            verbose=False, precision=prec)
+                            precision=prec, **kw)
 
-    for i in range(args.warmup):
-        _, info = step()
-        log(f"warmup {i} done: {info}")
-    classes = sorted(gpu.KERNEL_CLASSES)
-    if not args.no_ktimer:
-        for k in classes:
-            gpu.ktimer_collect(k)  # discard warm-up records
-            gpu.ktimer_enable(k, True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        _, info = step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    log(f"timed {args.steps} steps in {elapsed:.3f} s; info {info}")
-
-    kernels = {}
-    if not args.no_ktimer:
-        for k in classes:
-            ms, cnt, nbytes = gpu.ktimer_collect(k)
-            gpu.ktimer_enable(k, False)
-            if cnt:
-                kernels[gpu.KERNEL_CLASSES[k]] = dict(
-                    launches=cnt, total_ms=ms, avg_ms=ms / cnt,
-                    bytes_per_launch=nbytes / cnt,
-                    gbs=nbytes / (ms * 1e-3) / 1e9)
-    roofline = None
-    if kernels:
-        dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
-        kd = kernels[dom]
-        roofline = {
-            "kernel": dom,
-            "bound": "hbm",
-            "achieved": kd["gbs"],
-            "peak": PEAK_HBM_GBS,
-            "unit": "GB/s",
-            "frac": kd["gbs"] / PEAK_HBM_GBS,
-            # HBM bytes per launch (rocprofv3 counters), or None
-            "traffic": measured_traffic(args.workload, dom),
-            "bytes_per_launch": kd["bytes_per_launch"],
-            "avg_launch_ms": kd["avg_ms"],
-        }
+    elapsed, res, kernels = timed_steps(torch, gpu, step, args.steps,
+                                        args.warmup, classes, args.workload)
+    info = res[1]
+    value = w["slices"] * args.steps / elapsed
+    roofline = roofline_of(kernels, args.workload)
     valu = None
-    if ("toeplitz_cols" in kernels and not uneven and not autograd
-            and info and info.get("padded")):
-        import ctypes as C
+    cols = "toeplitz_cols_f16" if headline else "toeplitz_cols"
+    if cols in kernels and info and info.get("padded"):
         n = int(info["padded"])
-        L = C.c_int()
-        _lib.call("tc_toeplitz_fft_length", n, C.byref(L))
-        valu = valu_roofline(kernels["toeplitz_cols"], n, L.value,
-                             half=w.get("precision") == "mixed")
+        # mixed: FP32 intermediates, FP16 spectrum
+        valu = valu_roofline(kernels[cols], cols, n, fft_length(n), 8,
+                             2 if headline else 4)
+
+    fp32 = None
+    if headline:
+        out_mixed = res[0]
+        fsteps = max(1, min(args.fp32_steps, args.steps))
+
+        def fstep():
+            return gpu.mbir(sino, ang, center, w["iterations"], w["sigma"],
+                            precision="fp32")
+
+        fel, fres, fker = timed_steps(torch, gpu, fstep, fsteps, 1, classes,
+                                      "fp32")
+        d = (out_mixed.double() - fres[0].double()).norm()
+        fp32 = {
+            "slices_per_s": w["slices"] * fsteps / fel,
+            "steps": fsteps, "warmup": 1,
+            "ms_per_step": fel / fsteps * 1e3,
+            "speedup_mixed_vs_fp32": value / (w["slices"] * fsteps / fel),
+            "relL2_mixed_vs_fp32": float(d / fres[0].double().norm()),
+            "step": fres[1]["step"], "lipschitz": fres[1]["lipschitz"],
+            "roofline": roofline_of(fker, "plain-fp32-64x2048"),
+            "kernels": fker,
+        }
+        del out_mixed, fres
+
+    accuracy = None
+    if headline and not args.no_accuracy:
+        log("accuracy against the FP64 fixture ...")
+        accuracy = accuracy_vs_fp64(torch, gpu, phantom, w)
+        log(f"accuracy: {accuracy}")
 
     cpu = None
-    if (not args.no_cpu_baseline and not uneven and not autograd
+    if (not args.no_cpu_baseline and kind == "ellipses"
             and int(os.environ.get("RANK", "0")) == 0):
         log("cpu baseline sample ...")
-        cpu = cpu_baseline(sino.cpu().numpy(), ang, w, args.cpu_sample_slices,
+        ns = max(1, min(args.cpu_sample_slices, w["slices"]))
+        cpu = cpu_baseline(sino[:ns].cpu().numpy(), ang, w, w["slices"], ns,
                            args.cpu_sample_iters)
-        log(f"cpu baseline {cpu['value']:.4g} slices/s on {cpu['cores']} threads")
+        log(f"cpu baseline {cpu['value']:.4g} slices/s on {cpu['cores']} "
+            f"threads")
 
-    mixed = None
-    if not args.no_mixed and args.workload == "plain-fp32-32x1024":
-        if not uneven:
-            del sino
+    if headline and not args.no_extra:
+        del sino
         torch.cuda.empty_cache()
-        log("mixed precision vs FP32 on 64 x 2048^2 ...")
-        mixed = mixed_compare(torch, gpu, phantom, max(args.warmup, 1))
+        extra = extra_workloads(torch, gpu, phantom)
 
-    value = w["slices"] * args.steps / elapsed
+    mixed_dtype = "f16 storage (gradient, PSF spectrum) / f32 compute / " \
+                  "f64 line search"
     out = {
         "metric": "MBIR slices/s (mixed-prec vs plain FP32) at relL2≤1e-4 "
                   "vs FP64; HBM GB/s",
@@ -396,29 +533,31 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f16 storage / f32 compute" if w.get("precision") == "mixed"
-                 else "f32",
+        "dtype": mixed_dtype if headline else "f32",
         "data": "synthetic: seeded random-ellipse phantoms projected by the "
                 "library's NUFFT projector + Gaussian noise",
         "config": ({
             "workload": w["label"], "stacks": w["stacks"],
             "slices": w["slices"], "iterations": w["iterations"],
             "sigma": w["sigma"],
-        } if uneven else {
+        } if kind == "uneven" else {
             "workload": w["label"], "slices": w["slices"],
             "npix": w["npix"], "nproj": w["nproj"], "beta": w["beta"],
-        } if autograd else {
+        } if kind == "autograd" else {
             "workload": w["label"],
             "slices": w["slices"], "npix": w["npix"], "nproj": w["nproj"],
             "iterations": w["iterations"], "sigma": w["sigma"],
             "padded_width": int(info["padded"]),
             "accepted_iterations": int(info["iterations"]),
+            "step": info["step"],
         }),
         "roofline": roofline,
         "valu_roofline": valu,
         "kernels": kernels,
+        "fp32": fp32,
+        "accuracy": accuracy,
         "cpu_baseline": cpu,
-        "mixed_precision": mixed,
+        "extra": extra or None,
     }
     print(json.dumps(out), flush=True)
 

@@ -233,13 +233,12 @@ int tc_mbir2(const float* d_sino, const float* d_x0, float* d_recon,
 
 /* Mixed-precision form of tc_mbir2 (no counterpart in the reference, whose
  * Python path is FP32 throughout): the same pipeline and iteration, with
- * the gradient stored in FP16 under a per-slice power-of-two scale, the
- * Toeplitz intermediates and the PSF spectrum in FP16, FFT butterflies and
- * NUFFT spreading in FP32, the iterate, T x and b in FP32, line-search dots
- * in FP64, and T sol recomputed in FP32 every refresh_every accepted
- * iterations (iterative refinement of the T y recurrence; 0 = never).
- * On one 2048^2 slice, 30 iterations, sigma = 100, the result is 4.4e-5
- * relative L2 from FP64 with k = 5 and 1.0e-4 with k = 10.
+ * the gradient stored in FP16 under a per-slice power-of-two scale and the
+ * PSF spectrum in FP16, FFT butterflies, Toeplitz intermediates and NUFFT
+ * spreading in FP32, the iterate, T x and b in FP32, line-search dots in
+ * FP64, and T sol recomputed in FP32 every refresh_every accepted
+ * iterations (0 = never).  The step and T g use the same rounded gradient,
+ * so the T y recurrence drifts only by FP32 rounding.
  * lipschitz_iters > 0 takes L from that many power-iteration steps on T
  * (FP64 dots) instead of max(T 1).  Needs an FFT length with compile-time
  * passes (TC_ERR_INVALID otherwise).  Optional h_info[8] = tc_mbir2's six
@@ -306,7 +305,8 @@ int tc_mbir2_host(const float* h_sino, float* h_recon, int nslices, int nproj,
 
 /* Kernel classes: 0 Toeplitz rows (forward), 1 Toeplitz columns,
  * 2 Toeplitz rows (inverse), 3 MBIR gradient (stencil + assembly),
- * 4 MBIR commit, 5 standalone qGGMRF.  When enabled, launches of that class
+ * 4 MBIR commit, 5 standalone qGGMRF, 6-8 the FP16-storage Toeplitz passes
+ * of the mixed-precision form (rows forward, columns, rows inverse).  When enabled, launches of that class
  * are bracketed by HIP events on their stream. */
 int tc_ktimer_enable(int kernel_class, int on);
 

@@ -150,81 +150,116 @@ def qggmrf_add(x: np.ndarray, g: np.ndarray, sigma: float, p: float) -> None:
     lib().qggmrf_add_f32(_p(x), _p(g), S, R, Cc, float(sigma), float(p))
 
 
-def mbir2(sino: np.ndarray, angles, center: float, num_iters: int,
-          sigma: float, p: float = 1.2, timing: Optional[dict] = None):
-    """tomocam::mbir2 restated in FP32 on the CPU (reference control flow)."""
-    t0 = time.perf_counter()
-    sino = np.asarray(sino, dtype=np.float32)
-    maxv = sino.max()
-    if maxv == 0:
-        raise ValueError("sinogram has ZERO max value")
-    sino2 = sino / maxv
-    x0 = backproject(sino2, angles, fbp=True)
-    nrays = sino.shape[-1]
-    sino2 = O.preproc(sino2, center).astype(np.float32)
-    n2 = sino2.shape[-1]
-    x0 = O.pad2d(x0, n2 - nrays).astype(np.float32)
-    sinoT = backproject(sino2, angles)
-    sino_norm = np.float32(np.sum(sino2.astype(np.float64) ** 2))
-    g1 = Geometry(angles, n2, 2 * n2 - 1)
-    ones = np.ones((1, g1.u.size), dtype=np.complex64)
-    psf_arr = nufft1_real(ones, g1, 1.0)[0]
-    T = Toeplitz(psf_arr, n2)
-    L = float(T(np.ones((1, n2, n2), dtype=np.float32)).max())
-    step = np.float32(min(1.0, 1.0 / L))
-    t1 = time.perf_counter()
+class Mbir2CPU:
+    """tomocam::mbir2 restated in FP32 on the CPU (reference control flow),
+    split into the set-up (constructor) and resumable iterations."""
 
-    def grad(x):
-        gg = T(x) - sinoT
-        qggmrf_add(x, gg, sigma, p)
+    def __init__(self, sino: np.ndarray, angles, center: float,
+                 sigma: float, p: float = 1.2):
+        t0 = time.perf_counter()
+        sino = np.asarray(sino, dtype=np.float32)
+        maxv = sino.max()
+        if maxv == 0:
+            raise ValueError("sinogram has ZERO max value")
+        sino2 = sino / maxv
+        x0 = backproject(sino2, angles, fbp=True)
+        self.nrays = sino.shape[-1]
+        sino2 = O.preproc(sino2, center).astype(np.float32)
+        n2 = sino2.shape[-1]
+        x0 = O.pad2d(x0, n2 - self.nrays).astype(np.float32)
+        self.sinoT = backproject(sino2, angles)
+        self.sino_norm = np.float32(np.sum(sino2.astype(np.float64) ** 2))
+        t1 = time.perf_counter()
+        # the PSF, its spectrum and L: once per geometry (stack), not per
+        # slice
+        g1 = Geometry(angles, n2, 2 * n2 - 1)
+        ones = np.ones((1, g1.u.size), dtype=np.complex64)
+        psf_arr = nufft1_real(ones, g1, 1.0)[0]
+        self.T = Toeplitz(psf_arr, n2)
+        L = float(self.T(np.ones((1, n2, n2), dtype=np.float32)).max())
+        self.step0 = np.float32(min(1.0, 1.0 / L))
+        t2 = time.perf_counter()
+        self.sigma, self.p = sigma, p
+        self.sol = x0
+        self.x = x0
+        self.t = 1.0
+        self.tnew = 1.0
+        self.step = self.step0
+        self.timing = dict(setup_slices_s=t1 - t0, setup_psf_s=t2 - t1)
+
+    def _grad(self, x):
+        gg = self.T(x) - self.sinoT
+        qggmrf_add(x, gg, self.sigma, self.p)
         return gg
 
-    def err(x):
-        tx = T(x)
-        return math.sqrt(float(sino_norm + np.sum(x * tx, dtype=np.float32)
-                               - 2 * np.sum(x * sinoT, dtype=np.float32)))
+    def _err(self, x):
+        tx = self.T(x)
+        return math.sqrt(float(self.sino_norm
+                               + np.sum(x * tx, dtype=np.float32)
+                               - 2 * np.sum(x * self.sinoT, dtype=np.float32)))
 
-    sol = x0
-    x = sol
-    t = 1.0
-    tnew = 1.0
-    step0 = step
-    for _ in range(num_iters):
-        while True:
-            beta = np.float32(tnew * (1 / t - 1))
-            tnew = 0.5 * (math.sqrt(t ** 4 + 4 * t ** 2) - t ** 2)
-            y = sol + (sol - x) * beta
-            gr = grad(y)
-            sol = y - gr * step
-            fx = err(sol)
-            fy = err(y)
-            gy = 0.5 * step * float(np.sum(gr * gr, dtype=np.float32))
-            if fx > fy + gy:
-                step = np.float32(step * 0.9)
-            else:
-                step = step0
-                t = tnew
-                x = sol
-                break
-        err(sol)
-    out = O.postproc(sol, nrays)
-    t2 = time.perf_counter()
-    if timing is not None:
-        timing.update(setup_s=t1 - t0, iter_s=t2 - t1)
-    return out
+    def iterate(self, num_iters: int) -> None:
+        """``num_iters`` accepted iterations of Optimizer::run2
+        (src/optimize.h:96-154), four Toeplitz products per attempt."""
+        sol, x, t, tnew, step = self.sol, self.x, self.t, self.tnew, self.step
+        step0 = self.step0
+        for _ in range(num_iters):
+            while True:
+                beta = np.float32(tnew * (1 / t - 1))
+                tnew = 0.5 * (math.sqrt(t ** 4 + 4 * t ** 2) - t ** 2)
+                y = sol + (sol - x) * beta
+                gr = self._grad(y)
+                sol = y - gr * step
+                fx = self._err(sol)
+                fy = self._err(y)
+                gy = 0.5 * step * float(np.sum(gr * gr, dtype=np.float32))
+                if fx > fy + gy:
+                    step = np.float32(step * 0.9)
+                else:
+                    step = step0
+                    t = tnew
+                    x = sol
+                    break
+            self._err(sol)
+        self.sol, self.x, self.t, self.tnew, self.step = sol, x, t, tnew, step
+
+    def result(self) -> np.ndarray:
+        return O.postproc(self.sol, self.nrays)
+
+
+def mbir2(sino: np.ndarray, angles, center: float, num_iters: int,
+          sigma: float, p: float = 1.2) -> np.ndarray:
+    """tomocam::mbir2 (src/mbir2.cpp:40-145) in FP32 on the CPU."""
+    m = Mbir2CPU(sino, angles, center, sigma, p)
+    m.iterate(num_iters)
+    return m.result()
 
 
 def sample_rate(sino: np.ndarray, angles, center: float, num_iters: int,
-                sigma: float, sample_slices: int, sample_iters: int) -> dict:
-    """Times setup + ``sample_iters`` iterations on ``sample_slices`` slices
-    and extrapolates to ``num_iters`` iterations, linearly per
-    slice-iteration (slices are independent units of work)."""
+                sigma: float, sample_slices: int, sample_iters: int,
+                repeats: int = 3, total_slices: Optional[int] = None) -> dict:
+    """Throughput of the CPU restatement on a bounded sample: the set-up of
+    ``sample_slices`` slices once, one warm-up iteration, then ``repeats``
+    timed runs of ``sample_iters`` iterations, whose median is taken.
+    Extrapolated linearly to ``num_iters`` iterations per slice of the whole
+    stack (slices are independent units of work; the PSF is once per
+    stack)."""
+    total = total_slices or sino.shape[0]
     sample_slices = max(1, min(sample_slices, sino.shape[0]))
     sub = np.ascontiguousarray(sino[:sample_slices])
-    tm = {}
-    mbir2(sub, angles, center, sample_iters, sigma, timing=tm)
-    per_slice = (tm["setup_s"] / sample_slices
-                 + tm["iter_s"] / (sample_slices * sample_iters) * num_iters)
-    return dict(slices_per_s=1.0 / per_slice, setup_s=tm["setup_s"],
-                iter_s=tm["iter_s"], sample_slices=sample_slices,
-                sample_iters=sample_iters, threads=workers())
+    m = Mbir2CPU(sub, angles, center, sigma)
+    m.iterate(1)  # warm-up (FFT plans, thread pools)
+    runs = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        m.iterate(sample_iters)
+        runs.append(time.perf_counter() - t0)
+    it_s = float(np.median(runs)) / sample_iters  # per iteration, sample
+    tm = m.timing
+    per_slice = (tm["setup_slices_s"] / sample_slices
+                 + tm["setup_psf_s"] / total
+                 + it_s / sample_slices * num_iters)
+    return dict(slices_per_s=1.0 / per_slice, setup_s=tm["setup_slices_s"],
+                psf_s=tm["setup_psf_s"], iter_s=it_s, runs_s=runs,
+                sample_slices=sample_slices, sample_iters=sample_iters,
+                threads=workers())

@@ -30,11 +30,11 @@ def test_valu_roofline_arithmetic():
     import bench
     n, L = 1454, 2916
     H = L // 2 + 1
-    for half, ns in ((False, 8), (True, 16)):
+    for wtb, specb, ns in ((8, 4, 8), (4, 2, 16), (8, 2, 4)):
         # slices x (Wt in + out) + the spectrum, per launch
-        per = ns * (8 if half else 16) * n * H + (2 if half else 4) * H * L
-        r = bench.valu_roofline({"bytes_per_launch": per, "avg_ms": 1.0}, n,
-                                L, half=half)
+        per = ns * 2 * wtb * n * H + specb * H * L
+        r = bench.valu_roofline({"bytes_per_launch": per, "avg_ms": 1.0},
+                                "toeplitz_cols", n, L, wtb, specb)
         assert r["slices_per_launch"] == pytest.approx(ns)
         flops = ns * H * (10 * L * math.log2(L) + 2 * L)
         assert r["flops_per_launch"] == pytest.approx(flops)

@@ -488,3 +488,22 @@ def test_power_lipschitz_is_largest_eigenvalue():
     l20 = O.power_lipschitz(ps, n, 20)
     assert 0.5 * lam.max() < l20 <= lam.max() * (1 + 1e-12)
     assert abs(O.power_lipschitz(ps, n, 3000) - lam.max()) <= 1e-8 * lam.max()
+
+
+def test_cpu_baseline_port_matches_oracle():
+    """The benchmark's CPU baseline (FP32 restatement with w = 8 ES NUFFT
+    and C/OpenMP kernels) reconstructs what the FP64 oracle does, to the
+    FP32 bound (sigma = 100, 2 x 40^2, 20 iterations)."""
+    from oracle import cpu_port
+    P, N, S = 30, 40, 2
+    ang = np.arange(P) * np.pi / P
+    img = np.random.default_rng(9).random((S, N, N))
+    sino = O.project(img, ang, exact=True)
+    ref = O.mbir2(sino, ang, N / 2, 20, 100.0, exact=True)
+    got = cpu_port.mbir2(sino.astype(np.float32), ang, N / 2, 20, 100.0)
+    assert rel_l2(got, ref) < 1e-4
+    # split set-up / resumed iterations give the same result
+    m = cpu_port.Mbir2CPU(sino.astype(np.float32), ang, N / 2, 100.0)
+    m.iterate(7)
+    m.iterate(13)
+    assert np.array_equal(m.result(), got)

@@ -232,11 +232,10 @@ int tc_toeplitz_apply_mixed(tc_psf p, const float* d_x, float* d_out,
                               "plan; FFT length " + std::to_string(P.M()) +
                               " has none");
     hipStream_t s = as_stream(stream);
-    const long plane = (long)P.n() * P.n();
-    tc::DevBuf x16(2 * (size_t)nslices * plane);
+    tc::DevBuf x16(tc::half_pair_bytes(nslices, P.n(), P.n()));
     tc::DevBuf gsc(sizeof(float) * (size_t)nslices);
-    tc::to_half_slices(d_x, x16.as<void>(), gsc.as<float>(), nslices, plane,
-                       s);
+    tc::to_half_slices(d_x, x16.as<void>(), gsc.as<float>(), nslices, P.n(),
+                       P.n(), s);
     const int np = tc::toeplitz_num_partials(P, nslices);
     tc::DevBuf parts(sizeof(double) * (np + 1));
     P.apply_half(x16.as<void>(), gsc.as<float>(), d_out, nslices,

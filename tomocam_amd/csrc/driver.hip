@@ -160,8 +160,9 @@ Mbir2Result mbir2(const float* sino, const float* x0, float* recon, int S,
   // precision (TOMOCAM_HIP_AUTOTUNE=0 keeps the 144 MB rule)
   {
     const char* at = std::getenv("TOMOCAM_HIP_AUTOTUNE");
+    // (the mixed-precision loop keeps FP32 intermediates: same groups)
     if (o.autotune && !(at && at[0] == '0'))
-      psf.tune_groups(S, o.mixed, stream);
+      psf.tune_groups(S, false, stream);
   }
 
   // L = max(Re(A^H A 1)/n^3) = max(T 1); the qGGMRF Hessian term never

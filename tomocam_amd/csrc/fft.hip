@@ -52,6 +52,7 @@ int fft_next_smooth(int n, bool even) {
 }
 
 FFTLength::FFTLength(int L) {
+  ArenaPause no_arena;  // process-lifetime table, filled synchronously
   std::vector<int> r = fft_factor(L);
   plan_.L = L;
   plan_.nst = (int)r.size();

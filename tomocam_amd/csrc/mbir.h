@@ -29,10 +29,18 @@ void add_qggmrf_grad_free(const float* x, float* g, int S, int R, int C,
 double qggmrf_value_free(const float* x, int S, int R, int C, float sigma,
                          float p, hipStream_t stream);
 
-// x [S][plane] -> FP16 x * gsc[s] with gsc[s] the power of two that maps
-// max |x[s]| into (2^11, 2^12] (1 for an all-zero slice).
-void to_half_slices(const float* x, void* x16, float* gsc, int S, long plane,
-                    hipStream_t stream);
+// FP16 image stacks use a row-pair layout: word (s, q, c) of a
+// [S][ceil(R/2)][C] array of 32-bit words holds rows 2q and 2q + 1 of
+// column c as one half2 (the second half 0 past the last row), so the
+// Toeplitz row pass, which transforms rows in pairs, reads both with one
+// load.  Bytes of such a stack:
+size_t half_pair_bytes(int S, int R, int C);
+
+// x [S][R][C] -> FP16 x * gsc[s] in the row-pair layout, with gsc[s] the
+// power of two that maps max |x[s]| into (2^11, 2^12] (1 for an all-zero
+// slice).
+void to_half_slices(const float* x, void* x16, float* gsc, int S, int R,
+                    int C, hipStream_t stream);
 
 struct Run2Options {
   int num_iters = 100;
@@ -42,10 +50,11 @@ struct Run2Options {
   double sino_norm = 0.0;  // ||y||^2 of the (normalised, padded) sinogram
   int refresh_every = 10;  // recompute T(sol), T(x) every k accepted steps
   // Mixed precision: the gradient is stored in FP16 with a per-slice
-  // power-of-two scale and T g runs the FP16-storage Toeplitz passes; the
-  // state (sol, x, T sol, T x, b) stays FP32 and the dots FP64.  The
-  // refresh above recomputes T sol in FP32, bounding the drift that the
-  // FP16 T g leaves in the T y recurrence.
+  // power-of-two scale, and T g reads it and an FP16 copy of the spectrum
+  // (FP32 arithmetic and FP32 intermediates); the state (sol, x, T sol,
+  // T x, b) stays FP32 and the dots FP64.  The step and T g use the same
+  // rounded g, so the T y recurrence drifts only by FP32 rounding, as in
+  // the FP32 loop; the refresh above bounds that.
   bool mixed = false;
   // Replay the steady-state attempt from HIP graphs when the environment
   // sets TOMOCAM_HIP_GRAPHS=1 (off by default: it measured slower, see

@@ -29,6 +29,12 @@ constexpr float W_FACE = 0.0523f;
 constexpr float W_EDGE = 0.0370f;
 constexpr float W_CORNER = 0.0302f;
 
+// two FP16 values in one word (a in the low half)
+__device__ __forceinline__ unsigned pack_h2(__half a, __half b) {
+  return (unsigned)__half_as_ushort(a) |
+         ((unsigned)__half_as_ushort(b) << 16);
+}
+
 struct RhoParams {
   float inv_sigma;
   float inv_sigma2;
@@ -52,9 +58,10 @@ struct QArgs {
   const float* b;
   float* g;
   double* partials;    // NDOT per workgroup (MBIR mode)
-  // FP16 gradient (mixed precision): g16 = g * gsc[slice], and the largest
-  // |g| of each wave and plane into gmaxp
-  __half* g16;
+  // FP16 gradient (mixed precision): g16 = g * gsc[slice] in the row-pair
+  // layout (half_pair_bytes), and the largest |g| of each wave and plane
+  // into gmaxp
+  unsigned* g16;
   const float* gsc;
   float* gmaxp;
   int S, R, C;
@@ -261,7 +268,11 @@ __global__ void __launch_bounds__(256) qggmrf_kernel(QArgs a) {
     if (xout) {
       const long zo = (long)z * plane;
       float* gp = a.g + zo;
-      char* g16p = reinterpret_cast<char*>(a.g16 + (G16 ? zo : 0));
+      // FP16 g in the row-pair layout: plane z, pair q = row / 2, column x
+      unsigned* g16w =
+          G16 ? a.g16 + ((long)z * ((a.R + 1) / 2) + r0 / 2) * a.C + x
+              : nullptr;
+      __half hprev = __float2half_rn(0.f);
       float gs = 1.f, gsi = 1.f;
       if constexpr (G16) {
         gs = a.gsc[z];
@@ -289,7 +300,13 @@ __global__ void __launch_bounds__(256) qggmrf_kernel(QArgs a) {
               gm = fmaxf(gm, fabsf(gv));
               const __half h =
                   __float2half_rn(fminf(fmaxf(gv * gs, -65504.f), 65504.f));
-              *reinterpret_cast<__half*>(g16p + (o >> 1)) = h;
+              // r0 is even: rows r0 + j - 1 pair up as (odd j, even j); a
+              // last row of even index goes out with a zero partner
+              if ((j & 1) == 0)
+                g16w[(long)(j / 2 - 1) * a.C] = pack_h2(hprev, h);
+              else if (r0 + j == a.R)
+                g16w[(long)(j / 2) * a.C] = pack_h2(h, __float2half_rn(0.f));
+              hprev = h;
               gv = __half2float(h) * gsi;
             } else {
               at_byte(gp, o) = gv;
@@ -635,23 +652,38 @@ gscale_update_kernel(const float* __restrict__ gmaxp, int ntiles, int zchunk,
   }
 }
 
-__global__ void to_half_slices_kernel(const float* __restrict__ x,
-                                      __half* out, long plane, long n,
-                                      const float* __restrict__ gsc) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n)
-    out[i] = __float2half_rn(
-        fminf(fmaxf(x[i] * gsc[i / plane], -65504.f), 65504.f));
+size_t half_pair_bytes(int S, int R, int C) {
+  return sizeof(unsigned) * (size_t)S * (size_t)((R + 1) / 2) * (size_t)C;
 }
 
-void to_half_slices(const float* x, void* x16, float* gsc, int S, long plane,
-                    hipStream_t stream) {
+__global__ void to_half_slices_kernel(const float* __restrict__ x,
+                                      unsigned* out, int R, int C, long n,
+                                      const float* __restrict__ gsc) {
+  // one word (two rows of a column) per thread
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int np = (R + 1) / 2;
+  const long sq = i / C;
+  const int c = (int)(i - sq * C);
+  const int s = (int)(sq / np);
+  const int q = (int)(sq - (long)s * np);
+  const float g = gsc[s];
+  const float* xs = x + (long)s * R * C;
+  auto h = [&](int r) {
+    return r < R ? __float2half_rn(fminf(fmaxf(xs[(long)r * C + c] * g,
+                                               -65504.f), 65504.f))
+                 : __float2half_rn(0.f);
+  };
+  out[i] = pack_h2(h(2 * q), h(2 * q + 1));
+}
+void to_half_slices(const float* x, void* x16, float* gsc, int S, int R,
+                    int C, hipStream_t stream) {
   hipLaunchKernelGGL(gscale_init_kernel, dim3(S), dim3(256), 0, stream, x,
-                     plane, gsc);
+                     (long)R * C, gsc);
   TC_HIP(hipGetLastError());
-  const long n = (long)S * plane;
+  const long n = (long)S * ((R + 1) / 2) * C;
   hipLaunchKernelGGL(to_half_slices_kernel, dim3(cdiv(n, 256)), dim3(256), 0,
-                     stream, x, static_cast<__half*>(x16), plane, n, gsc);
+                     stream, x, static_cast<unsigned*>(x16), R, C, n, gsc);
   TC_HIP(hipGetLastError());
 }
 
@@ -812,14 +844,14 @@ Run2Stats run2_device(Psf& psf, const float* x0, const float* b, float* out,
     TC_REQUIRE(psf.has_fused(),
                "mixed precision needs a compile-time FFT plan; FFT length " +
                    std::to_string(psf.M()) + " has none");
-    bg16.alloc(sizeof(__half) * (size_t)N);
+    bg16.alloc(half_pair_bytes(S, n, n));
     bgsc.alloc(sizeof(float) * 2 * (size_t)S);
     bgmax.alloc(sizeof(float) * (size_t)qblk * QWAVES * qa.zchunk);
     bsat.alloc(sizeof(int));
     TC_HIP(hipMemsetAsync(bsat.as<void>(), 0, sizeof(int), stream));
     gsc = bgsc.as<float>();
     gsc_next = gsc + S;
-    qa.g16 = bg16.as<__half>();
+    qa.g16 = bg16.as<unsigned>();
     qa.gmaxp = bgmax.as<float>();
     hipLaunchKernelGGL(gscale_init_kernel, dim3(S), dim3(256), 0, stream, b,
                        (long)n * n, gsc);
@@ -893,7 +925,8 @@ Run2Stats run2_device(Psf& psf, const float* x0, const float* b, float* out,
         }
         const bool f =
             opt.mixed
-                ? psf.apply_commit_half(qa.g16, gsc, sc, S, tparts, stream)
+                ? psf.apply_commit_half(qa.g16, gsc, sc, S, tparts, stream,
+                                        /*wt16=*/false)
                 : allow_fused && psf.apply_commit(g, sc, S, tparts, stream);
         if (!f) psf.apply(g, nullptr, Tg, S, 0, tparts, stream);
         hipLaunchKernelGGL(reduce_dots_kernel, dim3(NDOT + 1), dim3(1024), 0,

@@ -557,6 +557,7 @@ __global__ void precorrect_kernel(const float* __restrict__ f, float2* fine,
 
 PolarNufft::PolarNufft(const float* h_angles, int nproj, int npix, int nmodes)
     : P_(nproj), N_(npix), nm_(nmodes) {
+  ArenaPause no_arena;  // cached plan, tables filled synchronously
   TC_REQUIRE(nproj >= 1 && npix >= 2 && nmodes >= 1, "NUFFT: bad geometry");
   TC_REQUIRE(nproj <= SP_LIST, "NUFFT: too many projection angles");
   TC_REQUIRE(npix < 65536, "NUFFT: too many samples per ray");
@@ -608,6 +609,7 @@ float2* PolarNufft::fine(int batch) {
 
 void PolarNufft::build_work(hipStream_t stream) {
   if (work_ready_) return;
+  ArenaPause no_arena;  // the plan's work lists, filled synchronously
   const int ntx = (int)cdiv(nf_, SP_T);
   const int ntiles = ntx * ntx;
   const int cidx = N_ / 2;
@@ -794,6 +796,7 @@ ray_dft_kernel(const void* __restrict__ in, void* __restrict__ out,
 }
 
 RayDft::RayDft(int N, int sign) : N_(N) {
+  ArenaPause no_arena;  // cached plan, tables filled synchronously
   TC_REQUIRE(N >= 1, "ray DFT: N must be >= 1");
   L_ = fft_next_smooth(2 * N - 1, false);
   const int c = N / 2;

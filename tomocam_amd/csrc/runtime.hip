@@ -66,6 +66,9 @@ ArenaScope::ArenaScope() {
 
 ArenaScope::~ArenaScope() { --t_arena.depth; }
 
+ArenaPause::ArenaPause() : saved_(t_arena.depth) { t_arena.depth = 0; }
+ArenaPause::~ArenaPause() { t_arena.depth = saved_; }
+
 static constexpr size_t kArenaGrain = size_t(2) << 20;  // 2 MiB
 
 // frees cached blocks of `a` (caller holds a->mu), largest first, until at

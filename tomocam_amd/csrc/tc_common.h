@@ -81,6 +81,21 @@ class ArenaScope {
   ArenaScope(const ArenaScope&) = delete;
   ArenaScope& operator=(const ArenaScope&) = delete;
 };
+// Turns the calling thread's arena off for its lifetime: long-lived objects
+// (FFT twiddle tables, NUFFT and ray-DFT plans) allocate fresh memory,
+// because they fill it with synchronous copies on the null stream, which
+// is not ordered after the pending work of the thread's stream that a
+// recycled block may still belong to.
+class ArenaPause {
+ public:
+  ArenaPause();
+  ~ArenaPause();
+  ArenaPause(const ArenaPause&) = delete;
+  ArenaPause& operator=(const ArenaPause&) = delete;
+
+ private:
+  int saved_;
+};
 // A cached block of at least `bytes` from the thread's arena when a scope is
 // active (its arena and capacity returned), else nullptr.
 void* arena_alloc(size_t bytes, Arena** owner, size_t* cap);
@@ -176,7 +191,11 @@ enum KernelClass : int {
   KC_MBIR_GRADIENT = 3,
   KC_MBIR_COMMIT = 4,
   KC_QGGMRF = 5,
-  KC_COUNT = 6,
+  // the FP16-storage Toeplitz passes (mixed precision), timed apart
+  KC_TOEPLITZ_ROWS_FWD_F16 = 6,
+  KC_TOEPLITZ_COLS_F16 = 7,
+  KC_TOEPLITZ_ROWS_INV_F16 = 8,
+  KC_COUNT = 9,
 };
 
 // Begins/ends a timed region for kernel class `kc` on `stream` if timing for

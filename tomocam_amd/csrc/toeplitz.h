@@ -70,17 +70,19 @@ class Psf {
 
   // Mixed-precision forms (compile-time FFT lengths only; false, launching
   // nothing, otherwise).  The input is FP16 with a per-slice power-of-two
-  // scale on the device: x[s] = g16[s] / gsc[s].  The passes compute in FP32
-  // and store their intermediates in FP16 (pass A output scaled by a power
-  // of two fixed by n, pass B output with a power-of-two scale per 32
-  // threads of a column workgroup, from their maximum), and read the
-  // spectrum from an FP16 copy.  The dot
-  // partials, T x and the committed step are in the input's true units.
+  // scale on the device: x[s] = g16[s] / gsc[s], and the spectrum is read
+  // from an FP16 copy; the arithmetic is FP32.  With `wt16` the passes also
+  // store their intermediates in FP16 (pass A output scaled by a power of
+  // two fixed by n, pass B output with a power-of-two scale per 32 threads
+  // of a column workgroup, from their maximum); without it the
+  // intermediates stay FP32.  The dot partials, T x and the committed step
+  // are in the input's true units.
   bool apply_half(const void* g16, const float* gsc, float* out, int nslices,
-                  double* dot_partials, hipStream_t stream);
+                  double* dot_partials, hipStream_t stream, bool wt16 = true);
   bool apply_commit_half(const void* g16, const float* gsc,
                          const StepCommit& c, int nslices,
-                         double* dot_partials, hipStream_t stream);
+                         double* dot_partials, hipStream_t stream,
+                         bool wt16 = true);
 
   // True when this FFT length has the compile-time passes (fused commit and
   // the mixed-precision forms).
@@ -107,7 +109,8 @@ class Psf {
   const ToeplitzStreams* streams_for(int nslices, int elem_bytes);
   ToeplitzStreams ts_{nullptr, nullptr, nullptr};
   bool run_half(const void* g16, const float* gsc, float* out, int nslices,
-                double* dot_partials, const StepCommit* c, hipStream_t stream);
+                double* dot_partials, const StepCommit* c, hipStream_t stream,
+                bool wt16);
   int n_, M_, H_, rp_;
   DevBuf arr_;
   DevBuf spec_;
@@ -122,11 +125,13 @@ class Psf {
 // Scales of the mixed-precision passes (see Psf::apply_half).
 struct ToeplitzHalf {
   const float* gsc;  // per-slice input scale (device)
+  float sinv;        // 2^-e, e the spectrum copy's exponent
   float* cinv;       // written by pass B (one per 32 threads of a column
                      // workgroup), read by pass C
   float wsc;         // pass A output scale: n * 2^16 * wsc <= 2^15
   float cbase;       // 1 / (2^e wsc), e the spectrum copy's exponent
-  float pconst;      // 1 / (2^e wsc^2): spectral dot back to input units
+  float pconst;      // spectral dot back to input units: 1 / (2^e wsc^2)
+                     // with FP16 intermediates, else 2^-e
 };
 
 // FFT length M (>= 2n - 1) of the Toeplitz product for an n x n image.

@@ -304,7 +304,7 @@ bool toeplitz_apply_ct_half(int M, const void* x16, float* out, void* wt,
                             int nslices, double* partials, hipStream_t stream,
                             double img_b, double spec_b,
                             const Psf::StepCommit* sc,
-                            const ToeplitzHalf& half,
+                            const ToeplitzHalf& half, bool wt16,
                             const ToeplitzStreams* ts);
 int toeplitz_half_group(int nslices, int H, int rp);
 int toeplitz_ct_scales(int M, int H);
@@ -332,7 +332,7 @@ __global__ void to_half_kernel(const float* __restrict__ v, __half* out,
 
 bool Psf::run_half(const void* g16, const float* gsc, float* out,
                    int nslices, double* dot_partials, const StepCommit* c,
-                   hipStream_t stream) {
+                   hipStream_t stream, bool wt16) {
   TC_REQUIRE(nslices >= 1, "Toeplitz: need at least one slice");
   TC_REQUIRE(g16 && gsc, "Toeplitz: null FP16 input or scales");
   if (toeplitz_ct_lpw(M_) == 0) return false;
@@ -361,7 +361,7 @@ bool Psf::run_half(const void* g16, const float* gsc, float* out,
   reserve(nslices);
   // pass B's output scales for two groups in flight (two-stream passes)
   const int group = 2 * toeplitz_half_group(nslices, H_, rp_);
-  if ((size_t)group > cinv_slices_) {
+  if (wt16 && (size_t)group > cinv_slices_) {
     cinv_.alloc(sizeof(float) * (size_t)group * toeplitz_ct_scales(M_, H_));
     cinv_slices_ = group;
   }
@@ -369,16 +369,17 @@ bool Psf::run_half(const void* g16, const float* gsc, float* out,
   (void)std::frexp((double)n_, &nb);  // 2^(nb-1) <= n < 2^nb
   ToeplitzHalf h;
   h.gsc = gsc;
+  h.sinv = std::ldexp(1.0f, -spec16_exp_);
   h.cinv = cinv_.as<float>();
   h.wsc = std::ldexp(1.0f, -(nb + 1));  // n * 2^16 * wsc < 2^15
   h.cbase = std::ldexp(1.0f, nb + 1 - spec16_exp_);
-  h.pconst = std::ldexp(1.0f, 2 * (nb + 1) - spec16_exp_);
+  h.pconst = wt16 ? std::ldexp(1.0f, 2 * (nb + 1) - spec16_exp_) : h.sinv;
   const double img_b = 4.0 * nslices * (double)n_ * n_;
   const double spec_b = 8.0 * nslices * (double)n_ * H_;
   return toeplitz_apply_ct_half(M_, g16, c ? c->Tdst : out, work_.as<void>(),
                                 spec16_.as<void>(), n_, H_, rp_, nslices,
                                 dot_partials, stream, img_b, spec_b, c, h,
-                                streams_for(nslices, 4));
+                                wt16, streams_for(nslices, wt16 ? 4 : 8));
 }
 
 bool Psf::has_fused() const { return toeplitz_ct_lpw(M_) > 0; }
@@ -463,17 +464,19 @@ int Psf::tune_groups(int nslices, bool half, hipStream_t stream) {
 }
 
 bool Psf::apply_half(const void* g16, const float* gsc, float* out,
-                     int nslices, double* dot_partials, hipStream_t stream) {
+                     int nslices, double* dot_partials, hipStream_t stream,
+                     bool wt16) {
   TC_REQUIRE(out != nullptr, "Toeplitz: null output");
-  return run_half(g16, gsc, out, nslices, dot_partials, nullptr, stream);
+  return run_half(g16, gsc, out, nslices, dot_partials, nullptr, stream, wt16);
 }
 
 bool Psf::apply_commit_half(const void* g16, const float* gsc,
                             const StepCommit& c, int nslices,
-                            double* dot_partials, hipStream_t stream) {
+                            double* dot_partials, hipStream_t stream,
+                            bool wt16) {
   TC_REQUIRE((c.xp == nullptr) == (c.Tx == nullptr),
              "Toeplitz: xp and Tx must both be given or both be null");
-  return run_half(g16, gsc, nullptr, nslices, dot_partials, &c, stream);
+  return run_half(g16, gsc, nullptr, nslices, dot_partials, &c, stream, wt16);
 }
 
 bool Psf::apply_commit(const float* g, const StepCommit& c, int nslices,

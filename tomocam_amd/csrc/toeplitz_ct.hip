@@ -157,7 +157,16 @@ __device__ __forceinline__ float2 unpack_half2(unsigned u) {
   return __half22float2(*reinterpret_cast<const __half2*>(&u));
 }
 
-template <class P, int CM, bool H16>
+// Storage of the passes (HM bits): HM_IN16 the input x is FP16 with a
+// per-slice power-of-two scale (x = x16 / gsc); HM_WT16 the intermediates
+// Wt are FP16 (pass A output at a fixed scale, pass B output with a scale
+// per 32 threads); HM_SPEC16 the spectrum is read from its FP16 copy.
+// Arithmetic is FP32 throughout.  Forms used: 0 (FP32), HM_IN16 |
+// HM_SPEC16 (the mixed-precision MBIR: FP16 gradient and spectrum, FP32
+// intermediates) and all three (the FP16-storage operator).
+constexpr int HM_IN16 = 1, HM_WT16 = 2, HM_SPEC16 = 4;
+
+template <class P, int CM, int HM>
 __global__ void __launch_bounds__(P::NT)
 ct_rows_fwd(const void* __restrict__ xv, void* __restrict__ wtv,
             const float2* __restrict__ tw, int n, int H, int rp, int nbps,
@@ -179,12 +188,9 @@ ct_rows_fwd(const void* __restrict__ xv, void* __restrict__ wtv,
   float2 v[E];
   float2 y[CM != 0 ? E : 1];
   const long oa = (long)s * n * n + (long)ra * n;
-  auto ld = [&](long o) -> float {
-    if constexpr (H16)
-      return __half2float(reinterpret_cast<const __half*>(xv)[o]);
-    else
-      return reinterpret_cast<const float*>(xv)[o];
-  };
+  // FP16 input: the row-pair layout, rows ra and ra + 1 in one word
+  const unsigned* xw = reinterpret_cast<const unsigned*>(xv) +
+                       ((long)s * ((n + 1) / 2) + ra / 2) * n;
   static_assert(R1 % 2 == 0, "plans need an even first radix");
 #pragma unroll
   for (int i = 0; i < E / R1; ++i)
@@ -194,8 +200,16 @@ ct_rows_fwd(const void* __restrict__ xv, void* __restrict__ wtv,
       const int e = i * R1 + r;
       float re = 0.f, im = 0.f;
       if ((!P::ZA || r < R1 / 2) && c < n) {
-        if (hasa) re = ld(oa + c);
-        if (hasb) im = ld(oa + n + c);
+        if constexpr ((HM & HM_IN16) != 0) {
+          if (hasa) {
+            const float2 f = unpack_half2(xw[c]);
+            re = f.x;
+            im = f.y;  // 0 past the last row
+          }
+        } else {
+          if (hasa) re = reinterpret_cast<const float*>(xv)[oa + c];
+          if (hasb) im = reinterpret_cast<const float*>(xv)[oa + n + c];
+        }
         if constexpr (CM != 0) {
           float ya = 0.f, yb = 0.f;
           if (hasa) {
@@ -214,7 +228,7 @@ ct_rows_fwd(const void* __restrict__ xv, void* __restrict__ wtv,
     }
   if constexpr (CM != 0) {
     // the step in the input's true units (FP16 input: x = g16 / gsc)
-    const float se = H16 ? cp.s / ha.gsc[s] : cp.s;
+    const float se = (HM & HM_IN16) ? cp.s / ha.gsc[s] : cp.s;
 #pragma unroll
     for (int i = 0; i < E / R1; ++i)
 #pragma unroll
@@ -234,7 +248,8 @@ ct_rows_fwd(const void* __restrict__ xv, void* __restrict__ wtv,
 #pragma unroll
     for (int r = 0; r < RS; ++r) ln[t + i * T + r * (L / RS)] = v[i * RS + r];
   __syncthreads();
-  const float hw = H16 ? 0.5f * ha.wsc : 0.5f;
+  constexpr bool W16 = (HM & HM_WT16) != 0;
+  const float hw = W16 ? 0.5f * ha.wsc : 0.5f;
   for (int q = threadIdx.x; q < LPW * H; q += P::NT) {
     const int k = q / LPW;
     const int l = q - k * LPW;
@@ -243,7 +258,7 @@ ct_rows_fwd(const void* __restrict__ xv, void* __restrict__ wtv,
       const float2 zk = sm[l * P::LS + k];
       const float2 zm = cconj(sm[l * P::LS + (k == 0 ? 0 : L - k)]);
       const float2 sp = cadd(zk, zm), df = csub(zk, zm);
-      if constexpr (H16) {
+      if constexpr (W16) {
         // |value| <= n * 2^16 * wsc <= 2^15: no overflow
         *reinterpret_cast<uint2*>(reinterpret_cast<unsigned*>(wtv) +
                                   wt_at(s, k, rr, H, rp)) =
@@ -263,7 +278,7 @@ ct_rows_fwd(const void* __restrict__ xv, void* __restrict__ wtv,
 // without leaving registers between the two transforms
 // ---------------------------------------------------------------------------
 
-template <class P, bool H16>
+template <class P, int HM>
 __global__ void __launch_bounds__(P::NTCL)
 ct_cols(void* __restrict__ wtv, const void* __restrict__ specv,
         const float2* __restrict__ tw, int n, int H, int rp, int nbps,
@@ -297,7 +312,7 @@ ct_cols(void* __restrict__ wtv, const void* __restrict__ specv,
       float2 w = make_float2(0.f, 0.f);
       if (r < R1 / 2 && active && idx < n) {
         const long o = wt_at(s, kc, idx, H, rp);
-        if constexpr (H16)
+        if constexpr ((HM & HM_WT16) != 0)
           w = unpack_half2(reinterpret_cast<const unsigned*>(wtv)[o]);
         else
           w = reinterpret_cast<const float2*>(wtv)[o];
@@ -314,7 +329,7 @@ ct_cols(void* __restrict__ wtv, const void* __restrict__ specv,
 #pragma unroll
       for (int r = 0; r < RS; ++r) {
         const long o = so + t + i * T + r * (L / RS);
-        if constexpr (H16)
+        if constexpr ((HM & HM_SPEC16) != 0)
           hs[i * RS + r] = __half2float(reinterpret_cast<const __half*>(specv)[o]);
         else
           hs[i * RS + r] = reinterpret_cast<const float*>(specv)[o];
@@ -339,7 +354,7 @@ ct_cols(void* __restrict__ wtv, const void* __restrict__ specv,
   }
   // the inverse's first stage needs no exchange (same element set)
   I::run(v, ln, t, active, tw);
-  if constexpr (H16) {
+  if constexpr ((HM & HM_WT16) != 0) {
     // Power-of-two scale per 32 consecutive threads (a half wave): their
     // largest kept value maps below 2^14.  A shuffle reduction within the
     // half wave (the workgroup is a whole number of waves, so every lane
@@ -392,8 +407,10 @@ ct_cols(void* __restrict__ wtv, const void* __restrict__ specv,
       }
   }
   if (partials) {
-    if constexpr (H16) {
-      const float gs = ha.gsc[s];
+    if constexpr (HM != 0) {
+      // back to the input's true units (input scale, spectrum and Wt
+      // scales folded into pconst)
+      const float gs = (HM & HM_IN16) ? ha.gsc[s] : 1.0f;
       acc *= (double)ha.pconst / ((double)gs * gs);
     }
     double tot = block_sum(acc, red);
@@ -407,7 +424,7 @@ ct_cols(void* __restrict__ wtv, const void* __restrict__ specv,
 //   2 / 3: out = Ty - s T x with Ty = Tsol (2) or Tsol + beta (Tsol - Tx) (3)
 // ---------------------------------------------------------------------------
 
-template <class P, int MODE, bool H16>
+template <class P, int MODE, int HM>
 __global__ void __launch_bounds__(P::NT)
 ct_rows_inv(const void* __restrict__ wtv, const float* bvec, float* out,
             const float2* __restrict__ tw, int n, int H, int rp, int nbps,
@@ -425,17 +442,18 @@ ct_rows_inv(const void* __restrict__ wtv, const float* bvec, float* out,
   constexpr int HC = L / 2 + 1;
   constexpr int NQ = (LPW * HC + P::NT - 1) / P::NT;
   float4 w4[NQ];
-  float cs[H16 ? NQ : 1], cs1[H16 ? NQ : 1];
+  constexpr bool W16 = (HM & HM_WT16) != 0;
+  float cs[W16 ? NQ : 1], cs1[W16 ? NQ : 1];
 #pragma unroll
   for (int j = 0; j < NQ; ++j) {
     const int q = threadIdx.x + j * P::NT;
     const int k = q / LPW;
     const int rr = r0 + 2 * (q - k * LPW);
     w4[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if constexpr (H16) cs[j] = cs1[j] = 0.f;
+    if constexpr (W16) cs[j] = cs1[j] = 0.f;
     if (q < LPW * H && rr < rp) {
       const long o = wt_at(s, k, rr, H, rp);
-      if constexpr (H16) {
+      if constexpr (W16) {
         const uint2 u = *reinterpret_cast<const uint2*>(
             reinterpret_cast<const unsigned*>(wtv) + o);
         const float2 a = unpack_half2(u.x), c = unpack_half2(u.y);
@@ -458,7 +476,7 @@ ct_rows_inv(const void* __restrict__ wtv, const float* bvec, float* out,
       const int k = q / LPW;
       const int l = q - k * LPW;
       float4 w = w4[j];
-      if constexpr (H16) {
+      if constexpr (W16) {
         w.x *= cs[j];
         w.y *= cs[j];
         w.z *= cs1[j];
@@ -485,8 +503,11 @@ ct_rows_inv(const void* __restrict__ wtv, const float* bvec, float* out,
   const int ra = r0 + 2 * line;
   const long oa = (long)s * n * n + (long)ra * n;
   const bool hasa = ra < n, hasb = ra + 1 < n;
-  // FP16 input: the transform is in units of gsc times the true values
-  const float gi = H16 ? 1.0f / ha.gsc[s] : 1.0f;
+  // FP16 input: the transform is in units of gsc times the true values;
+  // an FP16 spectrum with FP32 intermediates leaves its 2^e scale here
+  // (with FP16 intermediates pass B's scales include it)
+  float gi = (HM & HM_IN16) ? 1.0f / ha.gsc[s] : 1.0f;
+  if constexpr ((HM & HM_SPEC16) != 0 && !W16) gi *= ha.sinv;
   // Epilogue in NB batches: each batch's operands (b, or T sol and T x) are
   // all loaded before its stores, which the compiler could not otherwise
   // move past (out may alias Tsol).  Two batches instead of one take the
@@ -527,7 +548,7 @@ ct_rows_inv(const void* __restrict__ wtv, const float* bvec, float* out,
         const float2 q = v[e];
         float2 o = q;
         if constexpr (MODE == 0) {
-          if constexpr (H16) {
+          if constexpr (HM != 0) {
             o.x = q.x * gi;
             o.y = q.y * gi;
           }
@@ -635,13 +656,19 @@ int toeplitz_half_group(int nslices, int H, int rp) {
   return toeplitz_group_slices(nslices, H, rp, 4);
 }
 
-template <class P, bool H16>
+template <class P, int HM>
 static void run_ct(const void* x, const float* b, float* out, void* wt,
                    const void* spec, int n, int H, int rp, int nslices,
                    int mode, double* partials, hipStream_t stream0,
                    double img_b, double spec_b, const Psf::StepCommit* sc,
                    const ToeplitzHalf* half, const ToeplitzStreams* ts) {
   const float2* tw = fft_length(P::L).plan().tw;
+  // kernel-timer classes: the FP16-storage passes are timed apart
+  constexpr bool H16 = HM != 0;  // any FP16 storage: timed apart
+  constexpr bool W16 = (HM & HM_WT16) != 0;
+  const int kc_fwd = H16 ? KC_TOEPLITZ_ROWS_FWD_F16 : KC_TOEPLITZ_ROWS_FWD;
+  const int kc_cols = H16 ? KC_TOEPLITZ_COLS_F16 : KC_TOEPLITZ_COLS;
+  const int kc_inv = H16 ? KC_TOEPLITZ_ROWS_INV_F16 : KC_TOEPLITZ_ROWS_INV;
   const size_t smem = sizeof(float2) * P::LPW * P::LS;
   const size_t smemc = sizeof(float2) * P::LPWC * P::LSC;
   const int npairs = (n + 1) / 2;
@@ -650,12 +677,13 @@ static void run_ct(const void* x, const float* b, float* out, void* wt,
   const long img = (long)n * n;
   const int cm = sc ? (sc->xp ? 2 : 1) : 0;
   const int em = cm ? cm + 1 : mode;  // epilogue mode of pass C
-  const int group = toeplitz_group_slices(nslices, H, rp, H16 ? 4 : 8);
+  const int group = toeplitz_group_slices(nslices, H, rp, W16 ? 4 : 8);
   // per-slice algorithmic bytes (FP16 storage: the input and the
   // intermediates at half width)
   const double ib = img_b / nslices, sb = spec_b / nslices;
-  const double ibx = H16 ? 0.5 * ib : ib, sbw = H16 ? 0.5 * sb : sb;
-  const size_t xel = H16 ? 2 : 4;
+  const double ibx = (HM & HM_IN16) ? 0.5 * ib : ib;
+  const double sbw = W16 ? 0.5 * sb : sb;
+  const size_t xel = (HM & HM_IN16) ? 2 : 4;
   const bool dual = ts != nullptr && nslices > group;
   if (dual) {
     TC_HIP(hipEventRecord(ts->start, stream0));
@@ -668,7 +696,7 @@ static void run_ct(const void* x, const float* b, float* out, void* wt,
     // odd groups: the second stream and the second workspace half
     const bool odd = dual && (gi & 1);
     hipStream_t stream = odd ? ts->aux : stream0;
-    void* wtg = static_cast<char*>(wt) + (odd ? wslab * (H16 ? 4 : 8) : 0);
+    void* wtg = static_cast<char*>(wt) + (odd ? wslab * (W16 ? 4 : 8) : 0);
     CommitPtrs cp{};
     if (sc)
       cp = CommitPtrs{sc->sol + o, sc->xp ? sc->xp + o : nullptr,
@@ -685,47 +713,48 @@ static void run_ct(const void* x, const float* b, float* out, void* wt,
     const float* bg = b ? b + o : nullptr;
     float* og = (sc ? sc->Tdst : out) + o;
     {
-      ktimer_begin(KC_TOEPLITZ_ROWS_FWD, stream);
+      ktimer_begin(kc_fwd, stream);
       auto launch = [&](auto k) {
         allow_lds((const void*)k, smem);
         hipLaunchKernelGGL(k, dim3(nbr * ns), dim3(P::NT), smem, stream, xg,
                            wtg, tw, n, H, rp, nbr, cp, ha);
       };
-      if (cm == 0) launch(ct_rows_fwd<P, 0, H16>);
-      else if (cm == 1) launch(ct_rows_fwd<P, 1, H16>);
-      else launch(ct_rows_fwd<P, 2, H16>);
+      if (cm == 0) launch(ct_rows_fwd<P, 0, HM>);
+      else if (cm == 1) launch(ct_rows_fwd<P, 1, HM>);
+      else launch(ct_rows_fwd<P, 2, HM>);
       TC_HIP(hipGetLastError());
       // g in, Wt out; committing: sol (and xp) in, dst out
-      ktimer_end(KC_TOEPLITZ_ROWS_FWD, stream,
+      ktimer_end(kc_fwd, stream,
                  ns * (ibx + sbw + (cm ? (cm == 2 ? 3.0 : 2.0) * ib : 0.0)));
     }
     {
-      auto k = ct_cols<P, H16>;
+      auto k = ct_cols<P, HM>;
       allow_lds((const void*)k, smemc);
-      ktimer_begin(KC_TOEPLITZ_COLS, stream);
+      ktimer_begin(kc_cols, stream);
       hipLaunchKernelGGL(k, dim3(nbc * ns), dim3(P::NTCL), smemc, stream, wtg,
                          spec, tw, n, H, rp, nbc,
                          partials ? partials + (long)s0 * nbc : nullptr, ha);
       TC_HIP(hipGetLastError());
       // Wt in and out; the spectrum is read once per group
-      ktimer_end(KC_TOEPLITZ_COLS, stream,
-                 ns * 2.0 * sbw + (H16 ? 2.0 : 4.0) * (double)H * P::L);
+      ktimer_end(kc_cols, stream,
+                 ns * 2.0 * sbw +
+                     ((HM & HM_SPEC16) ? 2.0 : 4.0) * (double)H * P::L);
     }
     {
-      ktimer_begin(KC_TOEPLITZ_ROWS_INV, stream);
+      ktimer_begin(kc_inv, stream);
       auto launch = [&](auto k) {
         allow_lds((const void*)k, smem);
         hipLaunchKernelGGL(k, dim3(nbr * ns), dim3(P::NT), smem, stream, wtg,
                            bg, og, tw, n, H, rp, nbr, cp, ha);
       };
-      if (em == 0) launch(ct_rows_inv<P, 0, H16>);
-      else if (em == 1) launch(ct_rows_inv<P, 1, H16>);
-      else if (em == 2) launch(ct_rows_inv<P, 2, H16>);
-      else launch(ct_rows_inv<P, 3, H16>);
+      if (em == 0) launch(ct_rows_inv<P, 0, HM>);
+      else if (em == 1) launch(ct_rows_inv<P, 1, HM>);
+      else if (em == 2) launch(ct_rows_inv<P, 2, HM>);
+      else launch(ct_rows_inv<P, 3, HM>);
       TC_HIP(hipGetLastError());
       // Wt in, out written; plus b (mode 1) or Tsol (and Tx) when committing
       const double extra = em == 1 || em == 2 ? 1.0 : (em == 3 ? 2.0 : 0.0);
-      ktimer_end(KC_TOEPLITZ_ROWS_INV, stream, ns * (sbw + ib * (1.0 + extra)));
+      ktimer_end(kc_inv, stream, ns * (sbw + ib * (1.0 + extra)));
     }
   }
   if (dual) {
@@ -780,7 +809,7 @@ int toeplitz_ct_scales(int M, int H) {
   }
 }
 
-template <bool H16>
+template <int HM>
 static bool apply_ct(int M, const void* x, const float* b, float* out,
                      void* wt, const void* spec, int n, int H, int rp,
                      int nslices, int mode, double* partials,
@@ -788,7 +817,7 @@ static bool apply_ct(int M, const void* x, const float* b, float* out,
                      const Psf::StepCommit* sc, const ToeplitzHalf* half,
                      const ToeplitzStreams* ts) {
 #define TC_RUN_CT(PL)                                                        \
-  run_ct<PL, H16>(x, b, out, wt, spec, n, H, rp, nslices, mode, partials,   \
+  run_ct<PL, HM>(x, b, out, wt, spec, n, H, rp, nslices, mode, partials,    \
                   stream, img_b, spec_b, sc, half, ts);                     \
   return true
   switch (M) {
@@ -811,8 +840,8 @@ bool toeplitz_apply_ct(int M, const float* x, const float* b, float* out,
                        int nslices, int mode, double* partials,
                        hipStream_t stream, double img_b, double spec_b,
                        const Psf::StepCommit* sc, const ToeplitzStreams* ts) {
-  return apply_ct<false>(M, x, b, out, wt, spec, n, H, rp, nslices, mode,
-                         partials, stream, img_b, spec_b, sc, nullptr, ts);
+  return apply_ct<0>(M, x, b, out, wt, spec, n, H, rp, nslices, mode,
+                     partials, stream, img_b, spec_b, sc, nullptr, ts);
 }
 
 bool toeplitz_apply_ct_half(int M, const void* x16, float* out, void* wt,
@@ -820,10 +849,15 @@ bool toeplitz_apply_ct_half(int M, const void* x16, float* out, void* wt,
                             int nslices, double* partials, hipStream_t stream,
                             double img_b, double spec_b,
                             const Psf::StepCommit* sc,
-                            const ToeplitzHalf& half,
+                            const ToeplitzHalf& half, bool wt16,
                             const ToeplitzStreams* ts) {
-  return apply_ct<true>(M, x16, nullptr, out, wt, spec16, n, H, rp, nslices,
-                        0, partials, stream, img_b, spec_b, sc, &half, ts);
+  if (wt16)
+    return apply_ct<HM_IN16 | HM_WT16 | HM_SPEC16>(
+        M, x16, nullptr, out, wt, spec16, n, H, rp, nslices, 0, partials,
+        stream, img_b, spec_b, sc, &half, ts);
+  return apply_ct<HM_IN16 | HM_SPEC16>(M, x16, nullptr, out, wt, spec16, n, H,
+                                       rp, nslices, 0, partials, stream, img_b,
+                                       spec_b, sc, &half, ts);
 }
 
 }  // namespace tc

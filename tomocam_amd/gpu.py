@@ -236,6 +236,9 @@ KERNEL_CLASSES = {
     3: "mbir_gradient",
     4: "mbir_commit",
     5: "qggmrf",
+    6: "toeplitz_rows_fwd_f16",
+    7: "toeplitz_cols_f16",
+    8: "toeplitz_rows_inv_f16",
 }
 
 
@@ -369,10 +372,10 @@ def pad2d(img: torch.Tensor, padding: int) -> torch.Tensor:
     return out
 
 
-# FP32 refresh interval of the mixed-precision MBIR: every 10 iterations
-# leaves 1.0e-4 relative L2 against FP64 on one 2048^2 slice (30 iterations,
-# sigma = 100); every 5 halves the drift of the FP16 T g recurrence.
-MIXED_REFRESH = 5
+# FP32 refresh interval of T sol in the mixed-precision MBIR: T g uses the
+# same FP16-rounded gradient as the step, so the recurrence drifts by FP32
+# rounding only, and the FP32 loop's interval holds.
+MIXED_REFRESH = 10
 
 
 def mbir(sino: torch.Tensor, angles, center: float, num_iters: int = 100,
