@@ -131,6 +131,22 @@ int tc_toeplitz_fft_length(int n, int* fft_size);
 int tc_toeplitz_apply(tc_psf p, const float* d_x, float* d_out, int nslices,
                       double* h_xTx, void* stream);
 
+/* tc_toeplitz_apply without allocating (no counterpart in the reference;
+ * for callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * workspace (the row-transformed intermediate, the dot partials and their
+ * sum) lives in a 72 MiB array that the library reserves statically on
+ * each GPU for the life of the process; slices are processed as many at a
+ * time as fit.  A call needs one slice's intermediate, 8 (M/2 + 1) n' bytes
+ * rounded up (n' the width rounded up to even; 67.9 MB at n = 2908), plus
+ * 8 bytes per dot partial when h_xTx is requested; larger requests fail
+ * with TC_ERR_INVALID before any work.  The array is shared by every call
+ * on a GPU and keeps the last call's contents: calls that overlap on one
+ * GPU (concurrent streams or threads) race.  With h_xTx non-NULL the call
+ * synchronises `stream`. */
+int tc_toeplitz_apply_noalloc(tc_psf p, const float* d_x, float* d_out,
+                              int nslices, double* h_xTx, void* stream);
+
 /* Autotuner of the Toeplitz passes (no counterpart in the reference): times
  * one application for slice-group sizes of half, once and twice the default
  * (a group's intermediate stays in the memory-side cache between the three

@@ -35,6 +35,7 @@ SIGNATURES = {
     "tc_graph_replays": [C.POINTER(C.c_long)],
     "tc_toeplitz_apply": [vp, vp, vp, C.c_int, c_double_p, vp],
     "tc_toeplitz_apply_mixed": [vp, vp, vp, C.c_int, c_double_p, vp],
+    "tc_toeplitz_apply_noalloc": [vp, vp, vp, C.c_int, c_double_p, vp],
     "tc_psf_tune": [vp, C.c_int, C.c_int, c_int_p, vp],
     "tc_gradient2": [vp, vp, vp, vp, C.c_int, vp],
     "tc_function_value2": [vp, vp, vp, C.c_int, C.c_double, c_double_p, vp],

@@ -206,6 +206,79 @@ int tc_toeplitz_apply(tc_psf p, const float* d_x, float* d_out, int nslices,
   });
 }
 
+// ---------------------------------------------------------------------------
+// tc_toeplitz_apply_noalloc: tc_toeplitz_apply with its workspace (the
+// row-transformed intermediate Wt of up to kScratchBytes / (8 H rp) slices
+// at a time, the dot partials and their sum) in a statically allocated
+// device array instead of library allocations.  The array exists once per
+// GPU for the life of the process; host code reaches it only through the
+// device address hipGetSymbolAddress returns on that GPU.
+// ---------------------------------------------------------------------------
+
+constexpr size_t kScratchBytes = size_t(72) << 20;  // 72 MiB
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The address of `scratch` on the current device, looked up on the first
+// call there.
+void* scratch_on_current_device() {
+  int dev = 0;
+  TC_HIP(hipGetDevice(&dev));
+  TC_REQUIRE(dev >= 0 && dev < kMaxDevices, "no-alloc: device index " +
+                                                std::to_string(dev) +
+                                                " out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    TC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    TC_REQUIRE(p != nullptr, "no-alloc: scratch has no device address");
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
+int tc_toeplitz_apply_noalloc(tc_psf p, const float* d_x, float* d_out,
+                              int nslices, double* h_xTx, void* stream) {
+  return guarded([&] {
+    tc::Psf& P = get_psf(p);
+    TC_REQUIRE(d_x && d_out, "null argument");
+    TC_REQUIRE(nslices >= 1, "Toeplitz: need at least one slice");
+    hipStream_t s = as_stream(stream);
+    // layout: [dot partials (np) | their sum (1) | pad to 256 B | Wt]
+    const size_t np = h_xTx ? (size_t)tc::toeplitz_num_partials(P, nslices)
+                            : 0;
+    const size_t head = (sizeof(double) * (np + 1) + 255) / 256 * 256;
+    const size_t per = P.workspace_bytes_per_slice();
+    if (head + per > kScratchBytes)
+      throw tc::Error(TC_ERR_INVALID,
+                      "no-alloc Toeplitz apply: needs " +
+                          std::to_string(head + per) +
+                          " bytes of scratch (one slice of width " +
+                          std::to_string(P.n()) + " plus " +
+                          std::to_string(np) + " partial sums), has " +
+                          std::to_string(kScratchBytes));
+    unsigned char* base = static_cast<unsigned char*>(
+        scratch_on_current_device());
+    double* parts = reinterpret_cast<double*>(base);
+    P.apply_ws(d_x, d_out, nslices, h_xTx ? parts : nullptr, base + head,
+               kScratchBytes - head, s);
+    if (h_xTx) {
+      // every partial is written by the column pass: nothing to clear
+      tc::reduce_partials(parts, (long)np, parts + np, s);
+      TC_HIP(hipMemcpyAsync(h_xTx, parts + np, sizeof(double),
+                            hipMemcpyDeviceToHost, s));
+      TC_HIP(hipStreamSynchronize(s));
+    }
+  });
+}
+
 int tc_toeplitz_fft_length(int n, int* fft_size) {
   return guarded([&] {
     TC_REQUIRE(n >= 2 && fft_size, "Toeplitz: bad width");

@@ -48,6 +48,15 @@ class Psf {
   void apply(const float* x, const float* b, float* out, int nslices,
              int mode, double* dot_partials, hipStream_t stream);
 
+  // The same product (mode 0) with caller-provided workspace `ws` of
+  // ws_bytes (at least workspace_bytes_per_slice()): slice groups are sized
+  // to fit it, one stream, and nothing is allocated.  Partials as apply().
+  void apply_ws(const float* x, float* out, int nslices, double* dot_partials,
+                void* ws, size_t ws_bytes, hipStream_t stream);
+  size_t workspace_bytes_per_slice() const {
+    return sizeof(float2) * (size_t)H_ * rp_;
+  }
+
   // T g with a line-search step committed in the same passes (pass A reads
   // g, pass C produces T g):
   //   dst  = y - s g,      y  = sol + beta (sol - xp)
@@ -111,6 +120,9 @@ class Psf {
   bool run_half(const void* g16, const float* gsc, float* out, int nslices,
                 double* dot_partials, const StepCommit* c, hipStream_t stream,
                 bool wt16);
+  void apply_generic(const float* x, const float* b, float* out, int nslices,
+                     int mode, double* dot_partials, float2* wt,
+                     hipStream_t stream);
   int n_, M_, H_, rp_;
   DevBuf arr_;
   DevBuf spec_;

@@ -272,7 +272,8 @@ bool toeplitz_apply_ct(int M, const float* x, const float* b, float* out,
                        float2* wt, const float* spec, int n, int H, int rp,
                        int nslices, int mode, double* partials,
                        hipStream_t stream, double img_b, double spec_b,
-                       const Psf::StepCommit* cm, const ToeplitzStreams* ts);
+                       const Psf::StepCommit* cm, const ToeplitzStreams* ts,
+                       int group_cap = 0);
 int toeplitz_streams(int H, int rp, int elem_bytes, int nslices);
 int toeplitz_rule_group(int nslices, int H, int rp, int elem_bytes);
 void toeplitz_set_streams(int H, int rp, int elem_bytes, int nslices,
@@ -506,6 +507,16 @@ void Psf::apply(const float* x, const float* b, float* out, int nslices,
                         mode, dot_partials, stream, img_b, spec_b, nullptr,
                         streams_for(nslices, 8)))
     return;
+  apply_generic(x, b, out, nslices, mode, dot_partials, wt, stream);
+}
+
+// The runtime-plan passes (FFT lengths without a compile-time plan) on
+// workspace wt for nslices slices.
+void Psf::apply_generic(const float* x, const float* b, float* out,
+                        int nslices, int mode, double* dot_partials,
+                        float2* wt, hipStream_t stream) {
+  const double img_b = 4.0 * nslices * (double)n_ * n_;
+  const double spec_b = 8.0 * nslices * (double)n_ * H_;
   const FFTLength& F = fft_length(M_);
   const int npairs = (n_ + 1) / 2;
 
@@ -563,6 +574,34 @@ void Psf::apply(const float* x, const float* b, float* out, int nslices,
     TC_HIP(hipGetLastError());
     ktimer_end(KC_TOEPLITZ_ROWS_INV, stream,
                spec_b + img_b * (mode == 1 ? 2.0 : 1.0));
+  }
+}
+
+void Psf::apply_ws(const float* x, float* out, int nslices,
+                   double* dot_partials, void* ws, size_t ws_bytes,
+                   hipStream_t stream) {
+  TC_REQUIRE(nslices >= 1, "Toeplitz: need at least one slice");
+  const size_t per = workspace_bytes_per_slice();
+  TC_REQUIRE(ws && ws_bytes >= per, "Toeplitz: workspace of " +
+                                        std::to_string(ws_bytes) +
+                                        " bytes holds no slice (needs " +
+                                        std::to_string(per) + ")");
+  const int cap = (int)std::min<size_t>(ws_bytes / per, (size_t)nslices);
+  float2* wt = static_cast<float2*>(ws);
+  const double img_b = 4.0 * nslices * (double)n_ * n_;
+  const double spec_b = 8.0 * nslices * (double)n_ * H_;
+  if (toeplitz_apply_ct(M_, x, nullptr, out, wt, spectrum(), n_, H_, rp_,
+                        nslices, 0, dot_partials, stream, img_b, spec_b,
+                        nullptr, nullptr, cap))
+    return;
+  // runtime-plan passes, `cap` slices at a time
+  const long plane = (long)n_ * n_;
+  const int ppart = toeplitz_num_partials(*this, 1);
+  for (int s0 = 0; s0 < nslices; s0 += cap) {
+    const int ns = std::min(cap, nslices - s0);
+    apply_generic(x + s0 * plane, nullptr, out + s0 * plane, ns, 0,
+                  dot_partials ? dot_partials + (long)s0 * ppart : nullptr, wt,
+                  stream);
   }
 }
 

@@ -661,7 +661,8 @@ static void run_ct(const void* x, const float* b, float* out, void* wt,
                    const void* spec, int n, int H, int rp, int nslices,
                    int mode, double* partials, hipStream_t stream0,
                    double img_b, double spec_b, const Psf::StepCommit* sc,
-                   const ToeplitzHalf* half, const ToeplitzStreams* ts) {
+                   const ToeplitzHalf* half, const ToeplitzStreams* ts,
+                   int group_cap) {
   const float2* tw = fft_length(P::L).plan().tw;
   // kernel-timer classes: the FP16-storage passes are timed apart
   constexpr bool H16 = HM != 0;  // any FP16 storage: timed apart
@@ -677,7 +678,8 @@ static void run_ct(const void* x, const float* b, float* out, void* wt,
   const long img = (long)n * n;
   const int cm = sc ? (sc->xp ? 2 : 1) : 0;
   const int em = cm ? cm + 1 : mode;  // epilogue mode of pass C
-  const int group = toeplitz_group_slices(nslices, H, rp, W16 ? 4 : 8);
+  int group = toeplitz_group_slices(nslices, H, rp, W16 ? 4 : 8);
+  if (group_cap > 0) group = std::min(group, group_cap);
   // per-slice algorithmic bytes (FP16 storage: the input and the
   // intermediates at half width)
   const double ib = img_b / nslices, sb = spec_b / nslices;
@@ -815,10 +817,10 @@ static bool apply_ct(int M, const void* x, const float* b, float* out,
                      int nslices, int mode, double* partials,
                      hipStream_t stream, double img_b, double spec_b,
                      const Psf::StepCommit* sc, const ToeplitzHalf* half,
-                     const ToeplitzStreams* ts) {
+                     const ToeplitzStreams* ts, int group_cap = 0) {
 #define TC_RUN_CT(PL)                                                        \
   run_ct<PL, HM>(x, b, out, wt, spec, n, H, rp, nslices, mode, partials,    \
-                  stream, img_b, spec_b, sc, half, ts);                     \
+                  stream, img_b, spec_b, sc, half, ts, group_cap);          \
   return true
   switch (M) {
     case 750: TC_RUN_CT(Plan750);
@@ -839,9 +841,11 @@ bool toeplitz_apply_ct(int M, const float* x, const float* b, float* out,
                        float2* wt, const float* spec, int n, int H, int rp,
                        int nslices, int mode, double* partials,
                        hipStream_t stream, double img_b, double spec_b,
-                       const Psf::StepCommit* sc, const ToeplitzStreams* ts) {
+                       const Psf::StepCommit* sc, const ToeplitzStreams* ts,
+                       int group_cap) {
   return apply_ct<0>(M, x, b, out, wt, spec, n, H, rp, nslices, mode,
-                     partials, stream, img_b, spec_b, sc, nullptr, ts);
+                     partials, stream, img_b, spec_b, sc, nullptr, ts,
+                     group_cap);
 }
 
 bool toeplitz_apply_ct_half(int M, const void* x16, float* out, void* wt,
