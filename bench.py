@@ -397,6 +397,10 @@ def main():
     ap.add_argument("--iterations", type=int, default=None,
                     help="override the workload's iteration count (for "
                          "short counter-collection runs)")
+    ap.add_argument("--slices", type=int, default=None,
+                    help="override the workload's slice count (for "
+                         "counter-collection runs; stacks up to 2^26 "
+                         "pixels rasterise their phantoms on the host)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device-to-device copy, whose "
                          "FETCH_SIZE / WRITE_SIZE calibrate HBM counters")
@@ -415,6 +419,9 @@ def main():
     if args.iterations is not None:
         w["iterations"] = args.iterations
         w["label"] += f" (iterations overridden to {args.iterations})"
+    if args.slices is not None:
+        w["slices"] = args.slices
+        w["label"] += f" (slices overridden to {args.slices})"
     log(f"workload {args.workload}: {w['label']}")
     if args.copy_calibration:
         a = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -422,7 +429,9 @@ def main():
         torch.cuda.synchronize()
         torch.add(a, 1.0, out=c)  # elementwise kernel: 1 GiB in, 1 GiB out
         torch.cuda.synchronize()
-        log("copy calibration: 1 GiB read + 1 GiB written")
+        s = float(c.sum())  # reduction: 1 GiB in
+        log(f"copy calibration: 1 GiB read + 1 GiB written; 1 GiB read "
+            f"({s:.0f})")
         del a, c
     classes = [] if args.no_ktimer else sorted(gpu.KERNEL_CLASSES)
     headline = args.workload == HEADLINE
