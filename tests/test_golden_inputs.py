@@ -19,3 +19,15 @@ def test_mbir2_1024_golden_input_reproducible():
     np.testing.assert_allclose([sino.sum(), np.linalg.norm(sino)],
                                gold["sino_checksum"], rtol=1e-12)
     assert gold["sampled"].shape == (64, 64)
+
+
+def test_mbir2_2048_100it_fixture_fields():
+    """The 100-iteration 2048^2 fixture (the benchmark's FP64 accuracy
+    check) comes from the same input as the 30-iteration one and carries
+    the fields the benchmark reads."""
+    a = np.load(G.fixture_path(2048))
+    b = np.load(G.fixture_path(2048, 100))
+    assert int(b["iterations"]) == 100 and b["sampled"].shape == (64, 64)
+    np.testing.assert_array_equal(a["sino_checksum"], b["sino_checksum"])
+    assert float(a["lipschitz"]) == float(b["lipschitz"])
+    assert float(b["step"]) == 1.0
