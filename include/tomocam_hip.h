@@ -34,6 +34,7 @@ extern "C" {
 #define TC_ERR_VALUE 5    /* numerically invalid input (all-zero sinogram) */
 
 typedef struct tc_psf_s* tc_psf;
+typedef struct tc_psf64_s* tc_psf64;
 
 /* Message of the last failure on this thread ("" if none). */
 const char* tc_last_error(void);
@@ -314,6 +315,108 @@ int tc_mbir2_host(const float* h_sino, float* h_recon, int nslices, int nproj,
                   int nrays, const float* h_angles, float center,
                   int num_iters, float sigma, float tol, float xtol,
                   int verbose);
+
+/* ---------------------------------------------------------------------- */
+/* double precision: the reference's T = double instantiations              */
+/* (src/mbir2.cpp:158-167, src/mbir.cpp:124, src/radon.cpp:100,             */
+/* src/iradon.cpp:107, src/gradient2.cpp:107).  Arrays are float64 and every */
+/* kernel computes in FP64; the NUFFT kernel has width 16 (FINUFFT's width   */
+/* for the tolerance 1e-15 the reference plans double transforms with,     */
+/* src/nufft.h:219).  The qGGMRF derivative is evaluated in FP64 (the       */
+/* reference's d_pot_func is float-only, src/gpu/potential_function.cuh:    */
+/* 76-99, so its double regulariser carries FP32 rounding).  Widths up to   */
+/* 2047 (padded), where every FFT line fits one workgroup's LDS; larger     */
+/* widths fail with TC_ERR_INVALID.  Angles are float64 host arrays.        */
+/* ---------------------------------------------------------------------- */
+
+/* tomocam::project<double> (src/radon.cpp:70-100). */
+int tc_project_f64(const double* d_img, double* d_sino, int nslices, int npix,
+                   const double* h_angles, int nproj, void* stream);
+
+/* tomocam::backproject<double> (src/iradon.cpp:75-107). */
+int tc_backproject_f64(const double* d_sino, double* d_img, int nslices,
+                       int nproj, int npix, const double* h_angles, int fbp,
+                       void* stream);
+
+/* tomocam::gradient<double> (src/gradient.cpp:35-103). */
+int tc_gradient_f64(const double* d_x, const double* d_sinoT, double* d_g,
+                    int nslices, int npix, const double* h_angles, int nproj,
+                    void* stream);
+
+/* tomocam::function_value<double> (src/error.cpp:36-88); synchronises. */
+int tc_function_value_f64(const double* d_x, const double* d_sino,
+                          int nslices, int npix, const double* h_angles,
+                          int nproj, double* h_out, void* stream);
+
+/* PointSpreadFunction<double> (src/toeplitz.h:33-160) from the geometry;
+ * synchronises `stream`. */
+int tc_psf_create_f64(tc_psf64* out, const double* h_angles, int nproj,
+                      int npix, void* stream);
+int tc_psf_destroy_f64(tc_psf64 p);
+
+/* PointSpreadFunction<double>::convolve (src/toeplitz.h:95-101). */
+int tc_toeplitz_apply_f64(tc_psf64 p, const double* d_x, double* d_out,
+                          int nslices, void* stream);
+
+/* tomocam::gradient2<double> (src/gradient2.cpp:36-107). */
+int tc_gradient2_f64(tc_psf64 p, const double* d_x, const double* d_sinoT,
+                     double* d_g, int nslices, void* stream);
+
+/* tomocam::function_value2<double> (src/error2.cpp:40-94); synchronises. */
+int tc_function_value2_f64(tc_psf64 p, const double* d_x,
+                           const double* d_sinoT, int nslices, double sino_sq,
+                           double* h_out, void* stream);
+
+/* tomocam::add_total_var2<double> / add_total_var<double>
+ * (src/tv_update.cpp:41-106, src/tvd_update.cpp:41-117). */
+int tc_add_total_var2_f64(const double* d_x, double* d_g, int nslices,
+                          int nrows, int ncols, double sigma, double p,
+                          void* stream);
+int tc_add_total_var_f64(const double* d_x, double* d_g, int nslices,
+                         int nrows, int ncols, double sigma, double p,
+                         void* stream);
+
+/* FP64 forms of tc_qggmrf_grad_free / tc_qggmrf_value_free (the
+ * differentiable prior; no counterpart in the reference). */
+int tc_qggmrf_grad_free_f64(const double* d_x, double* d_g, int nslices,
+                            int nrows, int ncols, double sigma, double p,
+                            void* stream);
+int tc_qggmrf_value_free_f64(const double* d_x, int nslices, int nrows,
+                             int ncols, double sigma, double p,
+                             double* h_value, void* stream);
+
+/* tomocam::mbir2<double> (src/mbir2.cpp:40-145, :158-167): the Toeplitz
+ * MBIR with the FBP guess when d_x0 is NULL, p = 1.2.  Optional
+ * h_info[6] as tc_mbir2.  Synchronises. */
+int tc_mbir2_f64(const double* d_sino, const double* d_x0, double* d_recon,
+                 int nslices, int nproj, int nrays, const double* h_angles,
+                 double center, int num_iters, double sigma, double* h_info,
+                 void* stream);
+
+/* tomocam::mbir<double> (src/mbir.cpp:38-124): the NUFFT normal-operator
+ * MBIR; d_x0 required.  Optional h_info[6].  Synchronises. */
+int tc_mbir_f64(const double* d_sino, const double* d_x0, double* d_recon,
+                int nslices, int nproj, int nrays, const double* h_angles,
+                double center, int num_iters, double sigma, double* h_info,
+                void* stream);
+
+/* tomocam::xerror<double> (src/xerror.cpp:65-85): sum (a - b)^2;
+ * synchronises. */
+int tc_xerror_f64(const double* d_a, const double* d_b, long n, double* h_out,
+                  void* stream);
+
+/* tomocam::preproc<double> (src/preproc.cpp:73-108), as tc_preproc. */
+int tc_preproc_f64(const double* d_sino, double* d_out, int nslices,
+                   int nproj, int nrays, double center, int* h_npad,
+                   void* stream);
+
+/* tomocam::postproc<double> (src/postproc.cpp:69-102), as tc_postproc. */
+int tc_postproc_f64(const double* d_img, double* d_out, int nslices, int n,
+                    int npixels, void* stream);
+
+/* tomocam::pad2d<double> (src/padding.cpp:37-91), as tc_pad2d. */
+int tc_pad2d_f64(const double* d_img, double* d_out, int nslices, int n,
+                 int padding, void* stream);
 
 /* ---------------------------------------------------------------------- */
 /* instrumentation                                                          */

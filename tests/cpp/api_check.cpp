@@ -1,7 +1,8 @@
 // Exercises the tomocam:: C++ layer (include/tomocam/tomocam.hpp) the way
 // the reference's test drivers do (NPRandom inputs, test/timer.h:40-56) and
-// writes the results as raw float32 files into argv[1] for the Python test
-// to compare with the oracle.
+// writes the results as raw float32 (and, for the double instantiations,
+// float64) files into argv[1] for the Python test to compare with the
+// oracle.
 #include <cmath>
 #include <cstdio>
 #include <random>
@@ -22,12 +23,65 @@ struct NPRandom {
   }
 };
 
-void dump(const std::string& dir, const char* name, const float* p, size_t n) {
+template <typename T>
+void dump(const std::string& dir, const char* name, const T* p, size_t n) {
   std::string path = dir + "/" + name;
   FILE* f = std::fopen(path.c_str(), "wb");
   if (!f) throw std::runtime_error("cannot write " + path);
-  std::fwrite(p, sizeof(float), n, f);
+  std::fwrite(p, sizeof(T), n, f);
   std::fclose(f);
+}
+
+// The T = double instantiations on a fresh NPRandom(5489) stream: outputs as
+// raw float64 files (*.f64).
+void double_forms(const std::string& dir) {
+  NPRandom rng;
+  tomocam::DArray<double> a(tomocam::dim3_t(2, 33, 40));
+  for (size_t i = 0; i < a.size(); ++i) a[i] = rng.rand();
+  tomocam::DArray<double> g(a.dims());
+  g.init(0.0);
+  tomocam::add_total_var2(a, g, 10.0, 1.2);
+  dump(dir, "tv_out.f64", g.begin(), g.size());
+
+  const int N = 32, P = 24;
+  std::vector<double> ang(P);
+  for (int i = 0; i < P; ++i) ang[i] = i * M_PI / P;
+  tomocam::DArray<double> img(tomocam::dim3_t(2, N, N));
+  for (size_t i = 0; i < img.size(); ++i) img[i] = rng.rand();
+  auto sino = tomocam::project(img, ang);
+  auto bp = tomocam::backproject(sino, ang, false);
+  dump(dir, "sino.f64", sino.begin(), sino.size());
+  dump(dir, "bp.f64", bp.begin(), bp.size());
+
+  tomocam::NUFFT::Grid<double> grid(P, N, ang.data(), 0);
+  std::vector<tomocam::PointSpreadFunction<double>> psfs;
+  psfs.emplace_back(grid);
+  tomocam::DArray<double> x2(img.dims());
+  for (size_t i = 0; i < x2.size(); ++i) x2[i] = rng.rand();
+  auto g2 = tomocam::gradient2(x2, bp, psfs);
+  dump(dir, "grad2.f64", g2.begin(), g2.size());
+  const double fv2 = tomocam::function_value2(x2, bp, psfs, sino.norm());
+  dump(dir, "fv2.f64", &fv2, 1);
+  const double xe = tomocam::xerror(img, x2);
+  dump(dir, "xerror.f64", &xe, 1);
+
+  auto pre = tomocam::preproc(sino, 13.0);
+  dump(dir, "preproc.f64", pre.begin(), pre.size());
+  auto pad = tomocam::pad2d(img, 10, tomocam::PadType::SYMMETRIC);
+  auto back = tomocam::postproc(pad, N);
+  double d = 0.0;
+  for (size_t i = 0; i < img.size(); ++i) d += std::fabs(back[i] - img[i]);
+  std::printf("double pad/crop round trip: %g\n", d);
+  if (d != 0.0) throw std::runtime_error("pad2d/postproc round trip");
+
+  auto rec = tomocam::mbir2<double>(std::nullopt, sino, ang, N / 2.0, 10,
+                                    10.0, 0.0, 0.0);
+  dump(dir, "mbir.f64", rec.begin(), rec.size());
+  tomocam::DArray<double> x0(img.dims());
+  x0.init(0.0);
+  auto rec1 = tomocam::mbir<double>(x0, sino, ang, N / 2.0, 5, 10.0, 0.0, 0.0);
+  dump(dir, "mbir_nufft.f64", rec1.begin(), rec1.size());
+  std::printf("double forms ok\n");
 }
 
 }  // namespace
@@ -115,6 +169,8 @@ int main(int argc, char** argv) {
       std::printf("mixed vs fp32 relL2 %.3e\n", rel);
       if (!(rel < 1e-4)) return 1;
     }
+
+    double_forms(dir);
 
     // an all-zero sinogram throws (no guess)
     tomocam::DArray<float> zero(sino.dims());

@@ -1,7 +1,8 @@
 """The tomocam:: C++ layer (include/tomocam/tomocam.hpp) end to end: the
 driver tests/cpp/api_check (built by __graft_entry__.build()) runs on the GPU
 with NPRandom(5489) inputs like the reference's test drivers; its outputs are
-compared with the oracle here.  Tolerances as in the per-operator tests."""
+compared with the oracle here.  Tolerances as in the per-operator tests
+(FP32 for T = float, 1e-10 for the T = double instantiations)."""
 
 import os
 import subprocess
@@ -55,3 +56,34 @@ def test_cpp_interface(torch_cuda, tmp_path):
     ref = O.mbir2(sino, ang, N / 2, 10, 10.0, exact=True)
     assert rel_l2(rec, ref) < 1e-4
     assert "zero-sinogram threw: 1" in r.stdout
+
+    # the T = double instantiations (FP64 kernels) on a fresh NPRandom
+    # stream, against the oracle to FP64 tolerances
+    assert "double forms ok" in r.stdout
+    ld64 = lambda n, shape: np.fromfile(tmp_path / n, dtype=np.float64).reshape(shape)
+    rs = O.NPRandom()
+    a = rs.rand(2 * 33 * 40).reshape(2, 33, 40)
+    assert rel_l2(ld64("tv_out.f64", a.shape), O.qggmrf_grad(a, 10.0, 1.2)) < 1e-12
+    img = rs.rand(2 * N * N).reshape(2, N, N)
+    ang = np.arange(P) * np.pi / P
+    sino = ld64("sino.f64", (2, P, N))
+    assert rel_l2(sino, O.project(img, ang, exact=True)) < 1e-10
+    bp = ld64("bp.f64", (2, N, N))
+    assert rel_l2(bp, O.backproject(sino, ang, exact=True)) < 1e-10
+    ps = O.psf(ang, N, exact=True)
+    x2 = rs.rand(2 * N * N).reshape(2, N, N)
+    g2 = ld64("grad2.f64", (2, N, N))
+    assert rel_l2(g2 + bp, O.toeplitz(x2, ps)) < 1e-10
+    fv2 = float(ld64("fv2.f64", (1,))[0])
+    ref = O.function_value2(x2, bp, ps, float(np.sum(sino ** 2)))
+    assert abs(fv2 - ref) <= 1e-10 * abs(ref)
+    xe = float(ld64("xerror.f64", (1,))[0])
+    assert abs(xe - np.sum((img - x2) ** 2)) <= 1e-13 * xe
+    pre = O.preproc(sino, 13.0)
+    assert np.array_equal(ld64("preproc.f64", pre.shape), pre)
+    rec = ld64("mbir.f64", (2, N, N))
+    assert rel_l2(rec, O.mbir2(sino, ang, N / 2, 10, 10.0, exact=True)) < 1e-10
+    rec1 = ld64("mbir_nufft.f64", (2, N, N))
+    ref1 = O.mbir_nufft(np.zeros((2, N, N)), sino, ang, N / 2, 5, 10.0,
+                        exact=True)
+    assert rel_l2(rec1, ref1) < 1e-10

@@ -75,6 +75,35 @@ SIGNATURES = {
                                    C.c_int, vp],
     "tc_mbir2_host": [vp, vp, C.c_int, C.c_int, C.c_int, vp, C.c_float,
                       C.c_int, C.c_float, C.c_float, C.c_float, C.c_int],
+    "tc_project_f64": [vp, vp, C.c_int, C.c_int, vp, C.c_int, vp],
+    "tc_backproject_f64": [vp, vp, C.c_int, C.c_int, C.c_int, vp, C.c_int,
+                           vp],
+    "tc_gradient_f64": [vp, vp, vp, C.c_int, C.c_int, vp, C.c_int, vp],
+    "tc_function_value_f64": [vp, vp, C.c_int, C.c_int, vp, C.c_int,
+                              c_double_p, vp],
+    "tc_psf_create_f64": [C.POINTER(vp), vp, C.c_int, C.c_int, vp],
+    "tc_psf_destroy_f64": [vp],
+    "tc_toeplitz_apply_f64": [vp, vp, vp, C.c_int, vp],
+    "tc_gradient2_f64": [vp, vp, vp, vp, C.c_int, vp],
+    "tc_function_value2_f64": [vp, vp, vp, C.c_int, C.c_double, c_double_p,
+                               vp],
+    "tc_add_total_var2_f64": [vp, vp, C.c_int, C.c_int, C.c_int, C.c_double,
+                              C.c_double, vp],
+    "tc_add_total_var_f64": [vp, vp, C.c_int, C.c_int, C.c_int, C.c_double,
+                             C.c_double, vp],
+    "tc_qggmrf_grad_free_f64": [vp, vp, C.c_int, C.c_int, C.c_int,
+                                C.c_double, C.c_double, vp],
+    "tc_qggmrf_value_free_f64": [vp, C.c_int, C.c_int, C.c_int, C.c_double,
+                                 C.c_double, c_double_p, vp],
+    "tc_mbir2_f64": [vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, C.c_double,
+                     C.c_int, C.c_double, c_double_p, vp],
+    "tc_mbir_f64": [vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, C.c_double,
+                    C.c_int, C.c_double, c_double_p, vp],
+    "tc_xerror_f64": [vp, vp, C.c_long, c_double_p, vp],
+    "tc_preproc_f64": [vp, vp, C.c_int, C.c_int, C.c_int, C.c_double,
+                       C.POINTER(C.c_int), vp],
+    "tc_postproc_f64": [vp, vp, C.c_int, C.c_int, C.c_int, vp],
+    "tc_pad2d_f64": [vp, vp, C.c_int, C.c_int, C.c_int, vp],
     "tc_alloc_stats": [c_long_p, c_long_p, c_double_p],
     "tc_release_cached": [],
     "tc_ktimer_enable": [C.c_int, C.c_int],

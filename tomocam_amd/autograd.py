@@ -14,8 +14,11 @@ The reference has no differentiable form; this module provides it:
   gradient is the reference stencil (``add_total_var2``); at the faces pairs
   leaving the volume are dropped, so the gradient is exact.
 
-Arithmetic is FP32 on the GPU (value sums in FP64).  float64 tensors are
-accepted and cast; gradients come back in the input's dtype.
+Arithmetic is FP32 on the GPU (value sums in FP64) for float32 and
+lower-precision inputs, which are cast to float32; float64 inputs run the
+FP64 kernels (``tc_project_f64``, ``tc_backproject_f64``,
+``tc_qggmrf_*_free_f64``) end to end.  Gradients come back in the input's
+dtype.
 """
 
 import torch
@@ -23,10 +26,13 @@ import torch
 from . import gpu
 
 
-def _f32(t: torch.Tensor) -> torch.Tensor:
+def _work(t: torch.Tensor, dtype: torch.dtype) -> torch.Tensor:
+    """t detached as the arithmetic dtype: float64 stays float64, anything
+    else becomes float32."""
     if not t.is_cuda:
         raise ValueError("tomocam_amd.autograd works on CUDA (HIP) tensors")
-    return t.detach().to(torch.float32).contiguous()
+    wt = torch.float64 if dtype == torch.float64 else torch.float32
+    return t.detach().to(wt).contiguous()
 
 
 class RadonFunction(torch.autograd.Function):
@@ -34,18 +40,19 @@ class RadonFunction(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, img: torch.Tensor, angles) -> torch.Tensor:
-        ang = gpu._angles(angles)
+        # kept in FP64; project/backproject round them for the FP32 kernels
+        ang = gpu._angles64(angles)
         ctx.angles = ang
         ctx.dtype = img.dtype
         ctx.two_d = img.dim() == 2
-        out = gpu.project(_f32(img), ang)
+        out = gpu.project(_work(img, img.dtype), ang)
         if ctx.two_d:
             out = out[0]
         return out.to(img.dtype)
 
     @staticmethod
     def backward(ctx, grad: torch.Tensor):
-        g = _f32(grad)
+        g = _work(grad, ctx.dtype)
         if ctx.two_d:
             g = g[None]
         img_grad = gpu.backproject(g, ctx.angles, fbp=False)
@@ -61,18 +68,18 @@ class QGGMRFFunction(torch.autograd.Function):
     def forward(ctx, x: torch.Tensor, sigma: float, p: float) -> torch.Tensor:
         if sigma <= 0:
             raise ValueError("sigma must be > 0")
-        x32 = _f32(x)
-        ctx.save_for_backward(x32)
+        xw = _work(x, x.dtype)
+        ctx.save_for_backward(xw)
         ctx.sigma, ctx.p, ctx.dtype = float(sigma), float(p), x.dtype
-        v = gpu.qggmrf_value_free(x32, ctx.sigma, ctx.p)
+        v = gpu.qggmrf_value_free(xw, ctx.sigma, ctx.p)
         return torch.tensor(v, dtype=x.dtype, device=x.device)
 
     @staticmethod
     def backward(ctx, grad: torch.Tensor):
-        (x32,) = ctx.saved_tensors
-        g = torch.zeros_like(x32)
-        gpu.qggmrf_grad_free(x32, g, ctx.sigma, ctx.p)
-        return (g * grad.to(torch.float32)).to(ctx.dtype), None, None
+        (xw,) = ctx.saved_tensors
+        g = torch.zeros_like(xw)
+        gpu.qggmrf_grad_free(xw, g, ctx.sigma, ctx.p)
+        return (g * grad.to(xw.dtype)).to(ctx.dtype), None, None
 
 
 def radon(img: torch.Tensor, angles) -> torch.Tensor:

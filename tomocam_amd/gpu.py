@@ -21,14 +21,25 @@ def _stream() -> C.c_void_p:
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
-def _dev(t: torch.Tensor, name: str) -> C.c_void_p:
+def _dev(t: torch.Tensor, name: str,
+         dtype: torch.dtype = torch.float32) -> C.c_void_p:
     if not isinstance(t, torch.Tensor) or not t.is_cuda:
         raise TypeError(f"{name} must be a CUDA tensor")
-    if t.dtype != torch.float32:
-        raise TypeError(f"{name} must be float32")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {str(dtype).replace('torch.', '')}")
     if not t.is_contiguous():
         raise ValueError(f"{name} must be contiguous")
     return C.c_void_p(t.data_ptr())
+
+
+def _f64(t: torch.Tensor) -> bool:
+    """float64 tensors take the library's FP64 entry points (the
+    reference's T = double instantiations)."""
+    return isinstance(t, torch.Tensor) and t.dtype == torch.float64
+
+
+def _angles64(angles) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(angles, dtype=np.float64).ravel())
 
 
 def _as3d(t: torch.Tensor) -> torch.Tensor:
@@ -146,6 +157,11 @@ def add_total_var2(x: torch.Tensor, g: torch.Tensor, sigma: float,
     if x3.shape != g3.shape:
         raise ValueError("x and g must have the same shape")
     S, R, Cc = x3.shape
+    if _f64(x3):
+        _lib.call("tc_add_total_var2_f64", _dev(x3, "x", torch.float64),
+                  _dev(g3, "g", torch.float64), S, R, Cc, float(sigma),
+                  float(p), _stream())
+        return g
     _lib.call("tc_add_total_var2", _dev(x3, "x"), _dev(g3, "g"), S, R, Cc,
               float(sigma), float(p), _stream())
     return g
@@ -159,6 +175,11 @@ def add_total_var(x: torch.Tensor, g: torch.Tensor, sigma: float,
     if x3.shape != g3.shape:
         raise ValueError("x and g must have the same shape")
     S, R, Cc = x3.shape
+    if _f64(x3):
+        _lib.call("tc_add_total_var_f64", _dev(x3, "x", torch.float64),
+                  _dev(g3, "g", torch.float64), S, R, Cc, float(sigma),
+                  float(p), _stream())
+        return g
     _lib.call("tc_add_total_var", _dev(x3, "x"), _dev(g3, "g"), S, R, Cc,
               float(sigma), float(p), _stream())
     return g
@@ -172,6 +193,11 @@ def qggmrf_grad_free(x: torch.Tensor, g: torch.Tensor, sigma: float,
     if x3.shape != g3.shape:
         raise ValueError("x and g must have the same shape")
     S, R, Cc = x3.shape
+    if _f64(x3):
+        _lib.call("tc_qggmrf_grad_free_f64", _dev(x3, "x", torch.float64),
+                  _dev(g3, "g", torch.float64), S, R, Cc, float(sigma),
+                  float(p), _stream())
+        return g
     _lib.call("tc_qggmrf_grad_free", _dev(x3, "x"), _dev(g3, "g"), S, R, Cc,
               float(sigma), float(p), _stream())
     return g
@@ -182,6 +208,10 @@ def qggmrf_value_free(x: torch.Tensor, sigma: float, p: float = 1.2) -> float:
     x3 = _as3d(x)
     S, R, Cc = x3.shape
     v = C.c_double()
+    if _f64(x3):
+        _lib.call("tc_qggmrf_value_free_f64", _dev(x3, "x", torch.float64), S,
+                  R, Cc, float(sigma), float(p), C.byref(v), _stream())
+        return v.value
     _lib.call("tc_qggmrf_value_free", _dev(x3, "x"), S, R, Cc, float(sigma),
               float(p), C.byref(v), _stream())
     return v.value
@@ -279,6 +309,14 @@ def project(img: torch.Tensor, angles) -> torch.Tensor:
     S, N, N2 = img.shape
     if N != N2:
         raise ValueError("images must be square")
+    if _f64(img):
+        ang = _angles64(angles)
+        out = torch.empty((S, ang.size, N), dtype=torch.float64,
+                          device=img.device)
+        _lib.call("tc_project_f64", _dev(img, "img", torch.float64),
+                  _dev(out, "sino", torch.float64), S, N,
+                  ang.ctypes.data_as(C.c_void_p), int(ang.size), _stream())
+        return out
     ang = _angles(angles)
     out = torch.empty((S, ang.size, N), dtype=torch.float32, device=img.device)
     _lib.call("tc_project", _dev(img, "img"), _dev(out, "sino"), S, N,
@@ -292,6 +330,15 @@ def backproject(sino: torch.Tensor, angles, fbp: bool = False) -> torch.Tensor:
     [S, N, N]."""
     sino = _as3d(sino)
     S, P, N = sino.shape
+    if _f64(sino):
+        ang = _angles64(angles)
+        if ang.size != P:
+            raise ValueError(f"got {ang.size} angles for {P} projections")
+        out = torch.empty((S, N, N), dtype=torch.float64, device=sino.device)
+        _lib.call("tc_backproject_f64", _dev(sino, "sino", torch.float64),
+                  _dev(out, "img", torch.float64), S, P, N,
+                  ang.ctypes.data_as(C.c_void_p), int(bool(fbp)), _stream())
+        return out
     ang = _angles(angles)
     if ang.size != P:
         raise ValueError(f"got {ang.size} angles for {P} projections")
@@ -306,8 +353,15 @@ def gradient(x: torch.Tensor, sinoT: torch.Tensor, angles) -> torch.Tensor:
     x = _as3d(x)
     sinoT = _as3d(sinoT)
     S, N, _ = x.shape
-    ang = _angles(angles)
     g = torch.empty_like(x)
+    if _f64(x):
+        ang = _angles64(angles)
+        _lib.call("tc_gradient_f64", _dev(x, "x", torch.float64),
+                  _dev(sinoT, "sinoT", torch.float64),
+                  _dev(g, "g", torch.float64), S, N,
+                  ang.ctypes.data_as(C.c_void_p), int(ang.size), _stream())
+        return g
+    ang = _angles(angles)
     _lib.call("tc_gradient", _dev(x, "x"), _dev(sinoT, "sinoT"), _dev(g, "g"),
               S, N, ang.ctypes.data_as(C.c_void_p), int(ang.size), _stream())
     return g
@@ -318,21 +372,34 @@ def function_value(x: torch.Tensor, sino: torch.Tensor, angles) -> float:
     x = _as3d(x)
     sino = _as3d(sino)
     S, N, _ = x.shape
-    ang = _angles(angles)
     out = C.c_double()
+    if _f64(x):
+        ang = _angles64(angles)
+        _lib.call("tc_function_value_f64", _dev(x, "x", torch.float64),
+                  _dev(sino, "sino", torch.float64), S, N,
+                  ang.ctypes.data_as(C.c_void_p), int(ang.size), C.byref(out),
+                  _stream())
+        return out.value
+    ang = _angles(angles)
     _lib.call("tc_function_value", _dev(x, "x"), _dev(sino, "sino"), S, N,
               ang.ctypes.data_as(C.c_void_p), int(ang.size), C.byref(out),
               _stream())
     return out.value
 
 
+def _kind(t: torch.Tensor):
+    """(entry-point suffix, dtype) of the float32 or float64 form."""
+    return ("_f64", torch.float64) if _f64(t) else ("", torch.float32)
+
+
 def xerror(a: torch.Tensor, b: torch.Tensor) -> float:
     """sum (a - b)^2 (tomocam::xerror)."""
     if a.shape != b.shape:
         raise ValueError("shape mismatch")
+    sfx, dt = _kind(a)
     out = C.c_double()
-    _lib.call("tc_xerror", _dev(a, "a"), _dev(b, "b"), a.numel(),
-              C.byref(out), _stream())
+    _lib.call("tc_xerror" + sfx, _dev(a, "a", dt), _dev(b, "b", dt),
+              a.numel(), C.byref(out), _stream())
     return out.value
 
 
@@ -340,13 +407,14 @@ def preproc(sino: torch.Tensor, center: float) -> torch.Tensor:
     """Zero-pad and centre the sinogram (tomocam::preproc)."""
     sino = _as3d(sino)
     S, P, N = sino.shape
+    sfx, dt = _kind(sino)
     npad = C.c_int()
-    _lib.call("tc_preproc", None, None, S, P, N, float(center), C.byref(npad),
-              _stream())
-    out = torch.empty((S, P, N + 2 * npad.value), dtype=torch.float32,
+    _lib.call("tc_preproc" + sfx, None, None, S, P, N, float(center),
+              C.byref(npad), _stream())
+    out = torch.empty((S, P, N + 2 * npad.value), dtype=dt,
                       device=sino.device)
-    _lib.call("tc_preproc", _dev(sino, "sino"), _dev(out, "out"), S, P, N,
-              float(center), C.byref(npad), _stream())
+    _lib.call("tc_preproc" + sfx, _dev(sino, "sino", dt), _dev(out, "out", dt),
+              S, P, N, float(center), C.byref(npad), _stream())
     return out
 
 
@@ -354,10 +422,10 @@ def postproc(img: torch.Tensor, npixels: int) -> torch.Tensor:
     """Crop the padded reconstruction (tomocam::postproc)."""
     img = _as3d(img)
     S, n, _ = img.shape
-    out = torch.empty((S, npixels, npixels), dtype=torch.float32,
-                      device=img.device)
-    _lib.call("tc_postproc", _dev(img, "img"), _dev(out, "out"), S, n,
-              int(npixels), _stream())
+    sfx, dt = _kind(img)
+    out = torch.empty((S, npixels, npixels), dtype=dt, device=img.device)
+    _lib.call("tc_postproc" + sfx, _dev(img, "img", dt), _dev(out, "out", dt),
+              S, n, int(npixels), _stream())
     return out
 
 
@@ -365,10 +433,11 @@ def pad2d(img: torch.Tensor, padding: int) -> torch.Tensor:
     """Symmetric zero padding by ``padding`` in total (tomocam::pad2d)."""
     img = _as3d(img)
     S, n, _ = img.shape
+    sfx, dt = _kind(img)
     m = n + abs(int(padding))
-    out = torch.empty((S, m, m), dtype=torch.float32, device=img.device)
-    _lib.call("tc_pad2d", _dev(img, "img"), _dev(out, "out"), S, n,
-              int(padding), _stream())
+    out = torch.empty((S, m, m), dtype=dt, device=img.device)
+    _lib.call("tc_pad2d" + sfx, _dev(img, "img", dt), _dev(out, "out", dt),
+              S, n, int(padding), _stream())
     return out
 
 
@@ -391,13 +460,24 @@ def mbir(sino: torch.Tensor, angles, center: float, num_iters: int = 100,
 
     ``precision="mixed"`` runs the mixed-precision form (FP16 gradient and
     Toeplitz intermediates, FP32 state and FFT arithmetic, FP64 dots, T sol
-    refreshed in FP32 every ``refresh_every`` iterations, default 5 there
-    and 10 in FP32: FP16 T g drifts faster); info then also
-    has ``saturated`` (slice-attempts whose FP16 gradient clipped) and
-    ``backtracks``.  ``lipschitz_iters > 0`` (mixed only) takes L from that
-    many power-iteration steps on T instead of max(T 1)."""
-    if precision not in ("fp32", "mixed"):
-        raise ValueError("precision must be 'fp32' or 'mixed'")
+    refreshed in FP32 every ``refresh_every`` iterations, default 10 in both
+    forms); info then also has ``saturated`` (slice-attempts whose FP16
+    gradient clipped) and ``backtracks``.  ``lipschitz_iters > 0`` (mixed
+    only) takes L from that many power-iteration steps on T instead of
+    max(T 1).
+
+    A float64 sinogram (or ``precision="fp64"``) runs tomocam::mbir2<double>
+    (``tc_mbir2_f64``): every kernel in FP64, T sol recomputed each
+    iteration, the reference's stopping rule; ``tol``/``xtol``/
+    ``refresh_every`` do not apply there."""
+    if precision not in ("fp32", "mixed", "fp64"):
+        raise ValueError("precision must be 'fp32', 'mixed' or 'fp64'")
+    if _f64(sino) or precision == "fp64":
+        # the FP64 instantiation (tomocam::mbir2<double>): float64 in and out
+        if precision == "mixed" or lipschitz_iters:
+            raise ValueError("float64 input runs the FP64 form")
+        return _mbir_f64("tc_mbir2_f64", sino, angles, center, num_iters,
+                         sigma, guess)
     if lipschitz_iters and precision != "mixed":
         raise ValueError("lipschitz_iters is a mixed-precision option")
     if refresh_every is None:
@@ -443,6 +523,9 @@ def mbir_nufft(sino: torch.Tensor, angles, center: float,
     without a PSF: sinogram [S, P, N] and initial guess x0 [S, N, N] (both
     device) -> (recon [S, N, N], info)."""
     sino = _as3d(sino)
+    if _f64(sino):
+        return _mbir_f64("tc_mbir_f64", sino, angles, center, num_iters,
+                         sigma, x0)
     S, P, N = sino.shape
     ang = _angles(angles)
     if ang.size != P:
@@ -460,6 +543,88 @@ def mbir_nufft(sino: torch.Tensor, angles, center: float,
     keys = ("lipschitz", "step", "sino_norm", "max_sino", "padded",
             "iterations")
     return out, dict(zip(keys, info.tolist()))
+
+
+def _mbir_f64(entry, sino, angles, center, num_iters, sigma, x0):
+    """tc_mbir2_f64 / tc_mbir_f64 on float64 device tensors."""
+    sino = _as3d(sino)
+    if sino.dtype != torch.float64:
+        sino = sino.double().contiguous()
+    S, P, N = sino.shape
+    ang = _angles64(angles)
+    if ang.size != P:
+        raise ValueError(f"got {ang.size} angles for {P} projections")
+    xp = None
+    if x0 is not None:
+        x0 = _as3d(x0)
+        if tuple(x0.shape) != (S, N, N):
+            raise ValueError(f"initial guess must be [{S}, {N}, {N}]")
+        x0 = x0.double().contiguous()
+        xp = _dev(x0, "x0", torch.float64)
+    out = torch.empty((S, N, N), dtype=torch.float64, device=sino.device)
+    info = np.zeros(6, dtype=np.float64)
+    _lib.call(entry, _dev(sino, "sino", torch.float64), xp,
+              _dev(out, "recon", torch.float64), S, P, N,
+              ang.ctypes.data_as(C.c_void_p), float(center), int(num_iters),
+              float(sigma), info.ctypes.data_as(_lib.c_double_p), _stream())
+    keys = ("lipschitz", "step", "sino_norm", "max_sino", "padded",
+            "iterations")
+    return out, dict(zip(keys, info.tolist()))
+
+
+class PointSpreadFunction64:
+    """PointSpreadFunction<double> (src/toeplitz.h:33-160): the FP64 Toeplitz
+    operator of one polar geometry, on float64 tensors."""
+
+    def __init__(self, angles, npix: int):
+        self._h = C.c_void_p()
+        ang = _angles64(angles)
+        _lib.call("tc_psf_create_f64", C.byref(self._h),
+                  ang.ctypes.data_as(C.c_void_p), int(ang.size), int(npix),
+                  _stream())
+        self.n = int(npix)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value:
+            try:
+                _lib.fn("tc_psf_destroy_f64")(h)
+            except Exception:
+                pass
+            self._h = C.c_void_p()
+
+    def _check(self, x, name):
+        x = _as3d(x)
+        if x.shape[-1] != self.n or x.shape[-2] != self.n:
+            raise ValueError(f"{name} must be [S, {self.n}, {self.n}]")
+        return x
+
+    def convolve(self, x: torch.Tensor) -> torch.Tensor:
+        x = self._check(x, "x")
+        out = torch.empty_like(x)
+        _lib.call("tc_toeplitz_apply_f64", self._h, _dev(x, "x", torch.float64),
+                  _dev(out, "out", torch.float64), x.shape[0], _stream())
+        return out
+
+    def gradient2(self, x: torch.Tensor, sinoT: torch.Tensor) -> torch.Tensor:
+        x = self._check(x, "x")
+        sinoT = self._check(sinoT, "sinoT")
+        g = torch.empty_like(x)
+        _lib.call("tc_gradient2_f64", self._h, _dev(x, "x", torch.float64),
+                  _dev(sinoT, "sinoT", torch.float64),
+                  _dev(g, "g", torch.float64), x.shape[0], _stream())
+        return g
+
+    def function_value2(self, x: torch.Tensor, sinoT: torch.Tensor,
+                        sino_sq: float) -> float:
+        x = self._check(x, "x")
+        sinoT = self._check(sinoT, "sinoT")
+        v = C.c_double()
+        _lib.call("tc_function_value2_f64", self._h,
+                  _dev(x, "x", torch.float64),
+                  _dev(sinoT, "sinoT", torch.float64), x.shape[0],
+                  float(sino_sq), C.byref(v), _stream())
+        return v.value
 
 
 def mbir_batch(stacks, num_iters: int = 100, sigma: float = 100.0,
