@@ -283,6 +283,26 @@ __global__ void __launch_bounds__(256) qggmrf_kernel(QArgs a) {
       const float* txp = a.Tx + zo;
       const float* sp = a.sol + zo;
       const float* xpp = a.xprev + zo;
+      // Every operand of the strip's rows is loaded before the first
+      // store: a store between two loads orders them (the pointers may
+      // alias for all the compiler knows), and loads inside the rows'
+      // bounds checks cannot be hoisted, so each would wait out its
+      // latency in turn.  The offsets are clamped into the plane, so rows
+      // past the last one read in-bounds values that go unused.
+      constexpr int QE = MODE == 1 ? QROWS + 1 : 1;
+      float bv[QE], tv[QE], dv[MODE == 1 && HAS_X ? QROWS + 1 : 1];
+      if constexpr (MODE == 1) {
+#pragma unroll
+        for (int j = 1; j <= QROWS; ++j) {
+          const unsigned o = ro[j];
+          bv[j] = at_byte(bp, o);
+          tv[j] = at_byte(tsp, o);
+          if constexpr (HAS_X) {
+            tv[j] = fmaf(a.beta, tv[j] - at_byte(txp, o), tv[j]);
+            dv[j] = at_byte(sp, o) - at_byte(xpp, o);
+          }
+        }
+      }
 #pragma unroll
       for (int j = 1; j <= QROWS; ++j) {
         if (r0 + j - 1 < a.R) {
@@ -291,10 +311,8 @@ __global__ void __launch_bounds__(256) qggmrf_kernel(QArgs a) {
             at_byte(gp, o) += gc[j];
           } else {
             const float v = vc[j];
-            const float bb = at_byte(bp, o);
-            float tyv = at_byte(tsp, o);
-            if constexpr (HAS_X)
-              tyv = fmaf(a.beta, tyv - at_byte(txp, o), tyv);
+            const float bb = bv[j];
+            const float tyv = tv[j];
             float gv = (tyv - bb) + gc[j];
             if constexpr (G16) {
               gm = fmaxf(gm, fabsf(gv));
@@ -316,7 +334,7 @@ __global__ void __launch_bounds__(256) qggmrf_kernel(QArgs a) {
             acc[2] += (double)gv * (tyv - bb);
             acc[3] += (double)gv * gv;
             if constexpr (HAS_X) {
-              const float d = at_byte(sp, o) - at_byte(xpp, o);
+              const float d = dv[j];
               acc[4] += (double)d * d;
               acc[5] += (double)d * gv;
             }

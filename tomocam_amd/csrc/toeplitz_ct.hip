@@ -120,7 +120,11 @@ using Plan4500 = TPlan<4500, 30, 2, Radices<10, 15, 30>, Radices<30, 15, 10>>;
 using Plan4608 = TPlan<4608, 24, 2, Radices<24, 24, 8>, Radices<8, 24, 24>, 1>;
 using Plan2304 = TPlan<2304, 24, 2, Radices<24, 24, 4>, Radices<4, 24, 24>, 2>;
 using Plan768 = TPlan<768, 24, 6, Radices<24, 8, 4>, Radices<4, 8, 24>, 6>;
-using Plan5832 = TPlan<5832, 18, 1, Radices<18, 18, 18>, Radices<18, 18, 18>>;
+#ifndef TC_P5832_LPW
+#define TC_P5832_LPW 1
+#endif
+using Plan5832 = TPlan<5832, 18, TC_P5832_LPW, Radices<18, 18, 18>,
+                       Radices<18, 18, 18>, 1>;
 
 // Wt element (slice s, column k1, row): layout [s][k1][rp], so the column
 // pass reads whole columns.  (A row-group-major layout, which makes the row
@@ -201,11 +205,10 @@ ct_rows_fwd(const void* __restrict__ xv, void* __restrict__ wtv,
       float re = 0.f, im = 0.f;
       if ((!P::ZA || r < R1 / 2) && c < n) {
         if constexpr ((HM & HM_IN16) != 0) {
-          if (hasa) {
-            const float2 f = unpack_half2(xw[c]);
-            re = f.x;
-            im = f.y;  // 0 past the last row
-          }
+          // the raw word (second half 0 past the last row), unpacked only
+          // once every load is issued: an unpack next to its load makes
+          // each load wait out its latency in turn
+          if (hasa) re = __uint_as_float(xw[c]);
         } else {
           if (hasa) re = reinterpret_cast<const float*>(xv)[oa + c];
           if (hasb) im = reinterpret_cast<const float*>(xv)[oa + n + c];
@@ -226,6 +229,10 @@ ct_rows_fwd(const void* __restrict__ xv, void* __restrict__ wtv,
       }
       v[e] = make_float2(re, im);
     }
+  if constexpr ((HM & HM_IN16) != 0) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = unpack_half2(__float_as_uint(v[e].x));
+  }
   if constexpr (CM != 0) {
     // the step in the input's true units (FP16 input: x = g16 / gsc)
     const float se = (HM & HM_IN16) ? cp.s / ha.gsc[s] : cp.s;
@@ -303,6 +310,8 @@ ct_cols(void* __restrict__ wtv, const void* __restrict__ specv,
   // are inactive
   const bool active = line < LPW && k1 < H;
   const int kc = active ? k1 : 0;
+  // the slice's FP16 input scale (for the dot), fetched up front
+  const float gsc_s = (HM & HM_IN16) ? ha.gsc[s] : 1.0f;
   float2 v[E];
 #pragma unroll
   for (int i = 0; i < E / R1; ++i)
@@ -312,27 +321,42 @@ ct_cols(void* __restrict__ wtv, const void* __restrict__ specv,
       float2 w = make_float2(0.f, 0.f);
       if (r < R1 / 2 && active && idx < n) {
         const long o = wt_at(s, kc, idx, H, rp);
+        // FP16: the raw word, unpacked once every load is issued
         if constexpr ((HM & HM_WT16) != 0)
-          w = unpack_half2(reinterpret_cast<const unsigned*>(wtv)[o]);
+          w.x = __uint_as_float(reinterpret_cast<const unsigned*>(wtv)[o]);
         else
           w = reinterpret_cast<const float2*>(wtv)[o];
       }
       v[i * R1 + r] = w;
     }
+  if constexpr ((HM & HM_WT16) != 0) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = unpack_half2(__float_as_uint(v[e].x));
+  }
   // the spectrum values this thread multiplies by, loaded up front so
   // their latency overlaps the forward transform
-  float hs[E];
+  // (an FP16 spectrum stays packed two to a register until its use, which
+  // keeps the pass within the FP32 form's register budget)
+  constexpr bool S16 = (HM & HM_SPEC16) != 0;
+  float hs[S16 ? 1 : E];
+  __half2 hs2[S16 ? (E + 1) / 2 : 1];
   {
     const long so = (long)kc * L;
 #pragma unroll
     for (int i = 0; i < E / RS; ++i)
 #pragma unroll
       for (int r = 0; r < RS; ++r) {
+        const int e = i * RS + r;
         const long o = so + t + i * T + r * (L / RS);
-        if constexpr ((HM & HM_SPEC16) != 0)
-          hs[i * RS + r] = __half2float(reinterpret_cast<const __half*>(specv)[o]);
-        else
-          hs[i * RS + r] = reinterpret_cast<const float*>(specv)[o];
+        if constexpr (S16) {
+          const __half h = reinterpret_cast<const __half*>(specv)[o];
+          if (e & 1)
+            hs2[e / 2].y = h;
+          else
+            hs2[e / 2].x = h;
+        } else {
+          hs[e] = reinterpret_cast<const float*>(specv)[o];
+        }
       }
   }
   float2* ln = sm + line * P::LSC;
@@ -345,7 +369,12 @@ ct_cols(void* __restrict__ wtv, const void* __restrict__ specv,
     for (int i = 0; i < E / RS; ++i)
 #pragma unroll
       for (int r = 0; r < RS; ++r) {
-        const float h = hs[i * RS + r];
+        const int e = i * RS + r;
+        float h;
+        if constexpr (S16)
+          h = __half2float((e & 1) ? hs2[e / 2].y : hs2[e / 2].x);
+        else
+          h = hs[e];
         const float2 q = v[i * RS + r];
         facc += h * (q.x * q.x + q.y * q.y);
         v[i * RS + r] = cscale(q, h);
@@ -410,8 +439,7 @@ ct_cols(void* __restrict__ wtv, const void* __restrict__ specv,
     if constexpr (HM != 0) {
       // back to the input's true units (input scale, spectrum and Wt
       // scales folded into pconst)
-      const float gs = (HM & HM_IN16) ? ha.gsc[s] : 1.0f;
-      acc *= (double)ha.pconst / ((double)gs * gs);
+      acc *= (double)ha.pconst / ((double)gsc_s * gsc_s);
     }
     double tot = block_sum(acc, red);
     if (threadIdx.x == 0) partials[b] = tot;
@@ -438,26 +466,32 @@ ct_rows_inv(const void* __restrict__ wtv, const float* bvec, float* out,
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const int s = b / nbps;
   const int r0 = (b - s * nbps) * 2 * LPW;
+  // the slice's FP16 input scale, fetched now so that its latency overlaps
+  // the transform instead of delaying the epilogue
+  const float gsc_s = (HM & HM_IN16) ? ha.gsc[s] : 1.0f;
   // every Wt load of the thread in flight at once, then the LDS writes
   constexpr int HC = L / 2 + 1;
   constexpr int NQ = (LPW * HC + P::NT - 1) / P::NT;
   float4 w4[NQ];
   constexpr bool W16 = (HM & HM_WT16) != 0;
   float cs[W16 ? NQ : 1], cs1[W16 ? NQ : 1];
+  // FP16 Wt: raw words, unpacked after every load is issued
+  uint2 u16[W16 ? NQ : 1];
 #pragma unroll
   for (int j = 0; j < NQ; ++j) {
     const int q = threadIdx.x + j * P::NT;
     const int k = q / LPW;
     const int rr = r0 + 2 * (q - k * LPW);
     w4[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if constexpr (W16) cs[j] = cs1[j] = 0.f;
+    if constexpr (W16) {
+      cs[j] = cs1[j] = 0.f;
+      u16[j] = make_uint2(0u, 0u);
+    }
     if (q < LPW * H && rr < rp) {
       const long o = wt_at(s, k, rr, H, rp);
       if constexpr (W16) {
-        const uint2 u = *reinterpret_cast<const uint2*>(
+        u16[j] = *reinterpret_cast<const uint2*>(
             reinterpret_cast<const unsigned*>(wtv) + o);
-        const float2 a = unpack_half2(u.x), c = unpack_half2(u.y);
-        w4[j] = make_float4(a.x, a.y, c.x, c.y);
         // rows rr and rr + 1 may lie in different scale groups (odd T)
         const int i0 = col_scale_index<P>(s, k, rr, H);
         const int i1 = col_scale_index<P>(s, k, rr + 1, H);
@@ -467,6 +501,13 @@ ct_rows_inv(const void* __restrict__ wtv, const float* bvec, float* out,
         w4[j] = *reinterpret_cast<const float4*>(
             reinterpret_cast<const float2*>(wtv) + o);
       }
+    }
+  }
+  if constexpr (W16) {
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) {
+      const float2 a = unpack_half2(u16[j].x), c = unpack_half2(u16[j].y);
+      w4[j] = make_float4(a.x, a.y, c.x, c.y);
     }
   }
 #pragma unroll
@@ -506,7 +547,7 @@ ct_rows_inv(const void* __restrict__ wtv, const float* bvec, float* out,
   // FP16 input: the transform is in units of gsc times the true values;
   // an FP16 spectrum with FP32 intermediates leaves its 2^e scale here
   // (with FP16 intermediates pass B's scales include it)
-  float gi = (HM & HM_IN16) ? 1.0f / ha.gsc[s] : 1.0f;
+  float gi = (HM & HM_IN16) ? 1.0f / gsc_s : 1.0f;
   if constexpr ((HM & HM_SPEC16) != 0 && !W16) gi *= ha.sinv;
   // Epilogue in NB batches: each batch's operands (b, or T sol and T x) are
   // all loaded before its stores, which the compiler could not otherwise
