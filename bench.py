@@ -81,6 +81,8 @@ PEAK_HBM_GBS = 8000.0
 TRAFFIC_FILES = {
     "plain-fp32-32x1024": "mbir_fp32_32x1024_hbm_traffic.json",
     "mbir-64x2048": "mbir_mixed_64x2048_hbm_traffic.json",
+    # the same counter runs hold the plain FP32 classes of that workload
+    "plain-fp32-64x2048": "mbir_mixed_64x2048_hbm_traffic.json",
 }
 
 
@@ -278,8 +280,8 @@ def accuracy_vs_fp64(torch, gpu, phantom, w):
     FP64 oracle's 100-iteration result (tests/golden/
     mbir2_2048_slice_100it.npz, made by tests/golden/make_mbir2_golden.py
     from the same phantom and noise seeds).  The input here is projected by
-    the library's FP32 projector, not the oracle's; its checksum agrees
-    with the fixture's to about 1e-6, far below the 1e-4 bound."""
+    the library's FP32 projector, the fixture's by the oracle's FP64 one;
+    the line reports the relative difference of their checksums."""
     path = os.path.join(ROOT, "tests", "golden", "mbir2_2048_slice_100it.npz")
     if not os.path.exists(path):
         return None
