@@ -780,19 +780,23 @@ static std::atomic<long> g_graph_replays{0};
 long graph_replays() { return g_graph_replays.load(); }
 
 // Instantiated graphs of one steady-state attempt, keyed by the buffer
-// roles they were captured with.
+// roles and the step they were captured with (the step is a kernel
+// argument of the capture; a different step is a miss and runs eagerly).
 struct AttemptGraphs {
   struct Entry {
     const float* sol;
     const float* Tsol;
     const float* gsc;
+    float step;
     bool fused;
     hipGraphExec_t exec;
   };
   std::vector<Entry> e;
-  Entry* find(const float* sol, const float* Tsol, const float* gsc) {
+  Entry* find(const float* sol, const float* Tsol, const float* gsc,
+              float step) {
     for (auto& x : e)
-      if (x.sol == sol && x.Tsol == Tsol && x.gsc == gsc) return &x;
+      if (x.sol == sol && x.Tsol == Tsol && x.gsc == gsc && x.step == step)
+        return &x;
     return nullptr;
   }
   ~AttemptGraphs() {
@@ -962,7 +966,8 @@ Run2Stats run2_device(Psf& psf, const float* x0, const float* b, float* out,
       // graphs off for the rest of the call.
       bool fused;
       AttemptGraphs::Entry* ge =
-          use_graphs && !has_x ? graphs.find(sol, Tsol, gsc) : nullptr;
+          use_graphs && !has_x ? graphs.find(sol, Tsol, gsc, (float)step)
+                               : nullptr;
       if (ge && !ge->exec) {
         ge->exec = capture_attempt(stream, enqueue);
         if (!ge->exec) use_graphs = false;
@@ -974,7 +979,7 @@ Run2Stats run2_device(Psf& psf, const float* x0, const float* b, float* out,
       } else {
         fused = enqueue();
         if (use_graphs && !has_x && !ge)
-          graphs.e.push_back({sol, Tsol, gsc, fused, nullptr});
+          graphs.e.push_back({sol, Tsol, gsc, (float)step, fused, nullptr});
       }
       if (opt.mixed) std::swap(gsc, gsc_next);
       ++st.toeplitz_applies;
