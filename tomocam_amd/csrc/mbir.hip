@@ -857,7 +857,7 @@ Run2Stats run2_device(Psf& psf, const float* x0, const float* b, float* out,
   bool use_graphs = opt.allow_graphs && genv && genv[0] == '1' &&
                     !ktimer_active() && psf.has_fused() &&
                     (allow_fused || opt.mixed) &&
-                    psf.single_stream(S, opt.mixed);
+                    psf.single_stream(S, false);  // FP32 intermediates
   AttemptGraphs graphs;
   LoopStream loop_stream(caller, use_graphs);
   const hipStream_t stream = loop_stream.s;
