@@ -116,7 +116,11 @@ def fft_length(n):
     return L.value
 
 
-def measured_traffic(workload, kernel_class):
+def measured_traffic(workload, kernel_class, bytes_per_launch):
+    """HBM bytes per launch: the class's counter-measured traffic /
+    algorithmic ratio (committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs)
+    times this run's algorithmic bytes per launch, so that it holds for the
+    slice-group size the autotuner picked here."""
     name = TRAFFIC_FILES.get(workload)
     path = os.path.join(ROOT, "profiles", name) if name else None
     if not path or not os.path.exists(path):
@@ -124,7 +128,7 @@ def measured_traffic(workload, kernel_class):
     with open(path) as f:
         t = json.load(f)
     c = t["classes"].get(kernel_class)
-    return None if c is None else c["traffic_bytes"]
+    return None if c is None else c["ratio"] * bytes_per_launch
 
 
 def log(msg):
@@ -269,10 +273,24 @@ def roofline_of(kernels, workload):
         "unit": "GB/s",
         "frac": kd["gbs"] / PEAK_HBM_GBS,
         # HBM bytes per launch (rocprofv3 counters), or None
-        "traffic": measured_traffic(workload, dom),
+        "traffic": measured_traffic(workload, dom, kd["bytes_per_launch"]),
         "bytes_per_launch": kd["bytes_per_launch"],
         "avg_launch_ms": kd["avg_ms"],
     }
+
+
+def step_roofline(kernels, elapsed, steps):
+    """Whole-step rate: the algorithmic HBM bytes of every timed kernel class
+    per step over the wall time of a step (set-up included).  Unlike the
+    per-kernel figure it credits kernels that overlap on two streams."""
+    if not kernels:
+        return None
+    nbytes = sum(k["bytes_per_launch"] * k["launches"] for k in
+                 kernels.values()) / steps
+    ach = nbytes / (elapsed / steps) / 1e9
+    return {"achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": ach / PEAK_HBM_GBS, "bytes_per_step": nbytes,
+            "classes": sorted(kernels)}
 
 
 def accuracy_vs_fp64(torch, gpu, phantom, w):
@@ -338,6 +356,7 @@ def extra_workloads(torch, gpu, phantom):
         "workload": w["label"], "slices_per_s": w["slices"] * 3 / el,
         "ms_per_call": el / 3 * 1e3,
         "roofline": roofline_of(ker, "plain-fp32-32x1024"),
+        "step_roofline": step_roofline(ker, el, 3),
         "accepted_iterations": int(res[1]["iterations"])}
     del sino
     w = dict(WORKLOADS["uneven-200"])
@@ -505,6 +524,7 @@ def main():
             "relL2_mixed_vs_fp32": float(d / fres[0].double().norm()),
             "step": fres[1]["step"], "lipschitz": fres[1]["lipschitz"],
             "roofline": roofline_of(fker, "plain-fp32-64x2048"),
+            "step_roofline": step_roofline(fker, fel, fsteps),
             "kernels": fker,
         }
         del out_mixed, fres
@@ -564,6 +584,7 @@ def main():
         }),
         "roofline": roofline,
         "valu_roofline": valu,
+        "step_roofline": step_roofline(kernels, elapsed, args.steps),
         "kernels": kernels,
         "fp32": fp32,
         "accuracy": accuracy,

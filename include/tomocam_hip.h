@@ -149,16 +149,20 @@ int tc_toeplitz_apply_noalloc(tc_psf p, const float* d_x, float* d_out,
                               int nslices, double* h_xTx, void* stream);
 
 /* Autotuner of the Toeplitz passes (no counterpart in the reference): times
- * one application for slice-group sizes of half, once and twice the default
- * (a group's intermediate stays in the memory-side cache between the three
- * passes; the default keeps it near 144 MB) on scratch data, FP32 or the
- * mixed-precision passes, and keeps the fastest for this width, FFT length
- * and slice count for the rest of the process.  (The environment variable
- * TOMOCAM_HIP_TOEPLITZ_STREAMS=2 alternates slice groups over a second
- * stream; it is not tuned.)  *group (optional) receives the choice, -1 if the shape was
- * tuned before, 0 if there is nothing to tune (no compile-time plan, or one
- * slice).  tc_mbir2 / tc_mbir2_mixed tune on first use unless the
- * environment sets TOMOCAM_HIP_AUTOTUNE=0.  Synchronises `stream`. */
+ * one application on scratch data (FP32 or the mixed-precision passes) for
+ * slice-group sizes of once, half and twice the default (a group's
+ * intermediate stays in the memory-side cache between the three passes; the
+ * default keeps it near 144 MB) on one stream, and for once and half the
+ * default with groups alternating over a second stream (one group's row
+ * passes overlap the next group's column pass).  Candidates are taken in
+ * that order and a later one must be at least 2 % faster to replace the
+ * best, so near-ties resolve the same way on every run.  The choice holds
+ * for this width, FFT length and slice count for the rest of the process;
+ * TOMOCAM_HIP_TOEPLITZ_STREAMS=1 or 2 overrides the stream count.  *group
+ * (optional) receives the group size, -1 if the shape was tuned before, 0
+ * if there is nothing to tune (no compile-time plan, or one slice).
+ * tc_mbir2 / tc_mbir2_mixed tune on first use unless the environment sets
+ * TOMOCAM_HIP_AUTOTUNE=0.  Synchronises `stream`. */
 int tc_psf_tune(tc_psf p, int nslices, int mixed, int* group, void* stream);
 
 /* Mixed-precision T x (no counterpart in the reference): x rounded to FP16

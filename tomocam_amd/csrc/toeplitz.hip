@@ -416,10 +416,14 @@ int Psf::tune_groups(int nslices, bool half, hipStream_t stream) {
   // latency-bound row passes overlap the next group's VALU-bound column
   // pass: 2.5-3 % faster at 64 x 2908^2).  The application is timed on the
   // caller's stream, which waits for the second one.
+  // Candidates in order of preference; a later one replaces the best so
+  // far only when at least kMargin faster, so near-ties resolve the same
+  // way on every run and the configuration is reproducible.
+  constexpr float kMargin = 0.02f;
   const int r = toeplitz_rule_group(nslices, H_, rp_, eb);
   std::vector<std::pair<int, int>> cands;
   for (int st : {1, 2})
-    for (int g : {std::max(1, r / 2), r, std::min(2 * r, nslices)}) {
+    for (int g : {r, std::max(1, r / 2), std::min(2 * r, nslices)}) {
       if (st == 2 && (g > r || 2 * g > nslices)) continue;
       bool seen = false;
       for (auto& c : cands) seen |= c.first == g && c.second == st;
@@ -435,7 +439,7 @@ int Psf::tune_groups(int nslices, bool half, hipStream_t stream) {
       toeplitz_set_group(H_, rp_, eb, nslices, g);
       toeplitz_set_streams(H_, rp_, eb, nslices, st);
       float ms = 0.f;
-      for (int rep = 0; rep < 2; ++rep) {  // first run warms caches
+      for (int rep = 0; rep < 3; ++rep) {  // first run warms caches
         TC_HIP(hipEventRecord(e0, stream));
         if (half)
           apply_half(x.as<void>(), gsc.as<float>(), out.as<float>(), nslices,
@@ -445,9 +449,11 @@ int Psf::tune_groups(int nslices, bool half, hipStream_t stream) {
                 stream);
         TC_HIP(hipEventRecord(e1, stream));
         TC_HIP(hipEventSynchronize(e1));
-        TC_HIP(hipEventElapsedTime(&ms, e0, e1));
+        float t = 0.f;
+        TC_HIP(hipEventElapsedTime(&t, e0, e1));
+        if (rep == 1 || (rep > 1 && t < ms)) ms = t;
       }
-      if (best == 0 || ms < best_ms) {
+      if (best == 0 || ms < (1.0f - kMargin) * best_ms) {
         best = g;
         best_st = st;
         best_ms = ms;
