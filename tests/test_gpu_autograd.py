@@ -13,7 +13,11 @@ FP64 oracle's P^T (P x - y).
   over the probes <= 1e-4.
 - The differentiation workload's shape (8 x 512^2, 360 angles, beta = 1e-3): the gradient
   matches the oracle on the first two slices (relL2 <= 1e-4; the prior
-  term uses the whole volume).
+  term uses the whole volume), and 64 random directional derivatives match
+  FP64 central finite differences (eps = 1e-6) of the loss evaluated by the
+  FP64 kernels, which are themselves pinned to the oracle to 1e-10
+  (tests/test_gpu_fp64.py): within 1e-4 |grad| for the FP32 gradient and
+  1e-6 |grad| for the FP64 one.
 """
 
 import numpy as np
@@ -131,3 +135,44 @@ def test_config2_shape(torch_cuda):
     data = O.backproject(O.project(x32[:2], ang) - y[:2], ang)
     prior = O.qggmrf_grad_free(x32, sigma, 1.2)[:2]
     assert rel_l2(g[:2], data + beta * prior) < 1e-4
+
+
+def test_config2_fd_probes(torch_cuda):
+    """64 FP64 central finite differences at the differentiation workload's
+    shape (8 x 512^2, 360 angles, beta = 1e-3, sigma = 0.1)."""
+    from tomocam_amd import autograd as AG
+    t = torch_cuda
+    S, N, P, beta, sigma, eps = 8, 512, 360, 1e-3, 0.1, 1e-6
+    ang = PH.angles(P).astype(np.float64)
+    x = np.stack([PH.random_ellipses(N, 2200 + s) for s in range(S)])
+    x = x + 0.05 * np.random.default_rng(3).random(x.shape)
+    ys = np.stack([PH.random_ellipses(N, 2300 + s) for s in range(S)])
+    yt = AG.radon(t.tensor(ys, dtype=t.float64, device="cuda"), ang).detach()
+    announce("autograd 8 x 512^2: 64 FP64 finite-difference probes")
+    x64 = t.tensor(x, dtype=t.float64, device="cuda")
+    grads = {}
+    for dt in (t.float32, t.float64):
+        xt = x64.to(dt).requires_grad_(True)
+        loss = AG.mbir_loss(xt, yt.to(dt), ang, beta=beta, sigma=sigma, p=1.2)
+        loss.backward()
+        grads[dt] = xt.grad.detach().double()
+    gen = t.Generator(device="cuda")
+    gen.manual_seed(5)
+    fd, got32, got64 = [], [], []
+    with t.no_grad():
+        for _ in range(64):
+            d = t.randn(x.shape, dtype=t.float64, device="cuda", generator=gen)
+            d /= t.linalg.norm(d)
+            lp = float(AG.mbir_loss(x64 + eps * d, yt, ang, beta=beta,
+                                    sigma=sigma, p=1.2))
+            lm = float(AG.mbir_loss(x64 - eps * d, yt, ang, beta=beta,
+                                    sigma=sigma, p=1.2))
+            fd.append((lp - lm) / (2 * eps))
+            got32.append(float(t.sum(grads[t.float32] * d)))
+            got64.append(float(t.sum(grads[t.float64] * d)))
+    fd, got32, got64 = np.array(fd), np.array(got32), np.array(got64)
+    scale = float(t.linalg.norm(grads[t.float64]))
+    print(f"max |fd - g.d| / |g|: fp32 {np.abs(got32 - fd).max() / scale:.2e}"
+          f", fp64 {np.abs(got64 - fd).max() / scale:.2e}")
+    assert np.all(np.abs(got32 - fd) <= 1e-4 * scale)
+    assert np.all(np.abs(got64 - fd) <= 1e-6 * scale)
