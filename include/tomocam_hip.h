@@ -165,6 +165,19 @@ int tc_toeplitz_apply_noalloc(tc_psf p, const float* d_x, float* d_out,
  * TOMOCAM_HIP_AUTOTUNE=0.  Synchronises `stream`. */
 int tc_psf_tune(tc_psf p, int nslices, int mixed, int* group, void* stream);
 
+/* Autotuner of the FFT length, and so of the compile-time radix plan (no
+ * counterpart in the reference): where two compile-time plans fit the
+ * width (within 10 % of the smallest smooth length >= 2n - 1, e.g. 750 and
+ * 768 for n = 350 ... 362, 2250 and 2304, 4500 and 4608), times one FP32
+ * application of each on `nslices` slices of scratch data and rebuilds the
+ * handle's spectrum at the faster one; the static rule's choice is kept
+ * unless the other is at least 2 % faster.  The choice holds for this width
+ * for the rest of the process (later handles and tc_toeplitz_fft_length
+ * use it).  *fft_size (optional) receives the handle's FFT length.
+ * tc_mbir2 / tc_mbir2_mixed run it on first use of a width unless
+ * TOMOCAM_HIP_AUTOTUNE=0.  Synchronises `stream`. */
+int tc_psf_tune_length(tc_psf p, int nslices, int* fft_size, void* stream);
+
 /* Mixed-precision T x (no counterpart in the reference): x rounded to FP16
  * under a per-slice power-of-two scale, the three passes computing in FP32
  * with FP16 intermediates and FP16 spectrum (the operator the mixed MBIR

@@ -121,3 +121,32 @@ def test_toeplitz_group_autotuner(torch_cuda):
     small = gpu.PointSpreadFunction(
         psf_array=torch.tensor(_even_psf(181, 3), dtype=torch.float32).cuda())
     assert small.tune(S) == 0  # FFT length 378: no compile-time passes
+
+
+def test_toeplitz_length_autotuner(torch_cuda):
+    """The FFT-length autotuner at a width where two compile-time plans fit
+    (n = 350: 750 = 10 15 5 and 768 = 24 8 4, both within 10 % of the
+    smallest smooth length 700): it keeps one of them, later handles of the
+    width get the same length, T x does not change beyond FP32 rounding
+    (the product does not depend on the length), and a width with a single
+    plan has nothing to tune."""
+    torch = torch_cuda
+    from tomocam_amd import gpu
+    n, S = 350, 3
+    ps = _even_psf(n, 21)
+    x = np.random.default_rng(4).standard_normal((S, n, n))
+    ref = O.toeplitz(x, ps)
+    pst = torch.tensor(ps, dtype=torch.float32).cuda()
+    xt = torch.tensor(x, dtype=torch.float32).cuda()
+    psf = gpu.PointSpreadFunction(psf_array=pst)
+    assert psf.fft_size == 768  # the static rule (768 measured cheaper)
+    announce("toeplitz FFT-length autotuner n=350")
+    M = psf.tune_length(S)
+    assert M in (750, 768) and psf.fft_size == M
+    assert rel_l2(psf.convolve(xt).cpu().numpy(), ref) < 2e-5
+    again = gpu.PointSpreadFunction(psf_array=pst)
+    assert again.fft_size == M
+    assert again.tune_length(S) == M
+    single = gpu.PointSpreadFunction(
+        psf_array=torch.tensor(_even_psf(340, 5), dtype=torch.float32).cuda())
+    assert single.tune_length(S) == 750  # only 750 fits n = 340

@@ -99,6 +99,7 @@ SIGNATURES = {
                      C.c_int, C.c_double, c_double_p, vp],
     "tc_mbir_f64": [vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, C.c_double,
                     C.c_int, C.c_double, c_double_p, vp],
+    "tc_psf_tune_length": [vp, C.c_int, C.POINTER(C.c_int), vp],
     "tc_xerror_f64": [vp, vp, C.c_long, c_double_p, vp],
     "tc_preproc_f64": [vp, vp, C.c_int, C.c_int, C.c_int, C.c_double,
                        C.POINTER(C.c_int), vp],

@@ -295,6 +295,15 @@ int tc_psf_tune(tc_psf p, int nslices, int mixed, int* group, void* stream) {
   });
 }
 
+int tc_psf_tune_length(tc_psf p, int nslices, int* fft_size, void* stream) {
+  return guarded([&] {
+    tc::Psf& P = get_psf(p);
+    TC_REQUIRE(nslices >= 1, "Toeplitz: need at least one slice");
+    P.tune_length(nslices, as_stream(stream));
+    if (fft_size) *fft_size = P.M();
+  });
+}
+
 int tc_toeplitz_apply_mixed(tc_psf p, const float* d_x, float* d_out,
                             int nslices, double* h_xTx, void* stream) {
   return guarded([&] {

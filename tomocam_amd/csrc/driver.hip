@@ -156,13 +156,16 @@ Mbir2Result mbir2(const float* sino, const float* x0, float* recon, int S,
   polar_psf(h_angles, P, n2, bpsf.as<float>(), stream);
   Psf psf(bpsf.as<float>(), n2, stream);
   bpsf.release();
-  // slice-group size of the Toeplitz passes, timed once per shape and
-  // precision (TOMOCAM_HIP_AUTOTUNE=0 keeps the 144 MB rule)
+  // FFT length (where two compile-time plans fit this width), then the
+  // slice groups and stream count of the Toeplitz passes, timed once per
+  // shape and precision (TOMOCAM_HIP_AUTOTUNE=0 keeps the static rules)
   {
     const char* at = std::getenv("TOMOCAM_HIP_AUTOTUNE");
     // (the mixed-precision loop keeps FP32 intermediates: same groups)
-    if (o.autotune && !(at && at[0] == '0'))
+    if (o.autotune && !(at && at[0] == '0')) {
+      psf.tune_length(S, stream);
       psf.tune_groups(S, false, stream);
+    }
   }
 
   // L = max(Re(A^H A 1)/n^3) = max(T 1); the qGGMRF Hessian term never

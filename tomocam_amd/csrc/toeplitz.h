@@ -105,6 +105,15 @@ class Psf {
   // or a single slice).
   int tune_groups(int nslices, bool half, hipStream_t stream);
 
+  // Autotuner of the FFT length (and so of the compile-time radix plan):
+  // where two compile-time plans are eligible for this width, times one
+  // FP32 application of each on scratch data and rebuilds the spectrum at
+  // the faster length; the choice holds for this width for the rest of the
+  // process (toeplitz_fft_length returns it).  Returns the length, -1 if
+  // the width was tuned before (the spectrum then follows that choice), 0
+  // if there is a single candidate.
+  int tune_length(int nslices, hipStream_t stream);
+
   // Grows the workspace for `nslices` slices.
   void reserve(int nslices);
 
@@ -120,6 +129,8 @@ class Psf {
   bool run_half(const void* g16, const float* gsc, float* out, int nslices,
                 double* dot_partials, const StepCommit* c, hipStream_t stream,
                 bool wt16);
+  // (re)builds the spectrum at FFT length M from the PSF array
+  void build_spectrum(int M, hipStream_t stream);
   void apply_generic(const float* x, const float* b, float* out, int nslices,
                      int mode, double* dot_partials, float2* wt,
                      hipStream_t stream);

@@ -2,7 +2,11 @@
 // Reference: src/toeplitz.h:51-159 (PointSpreadFunction ctor / convolve).
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
+#include <utility>
 #include <vector>
 
 #include "fft_device.h"
@@ -18,25 +22,54 @@ extern const int kToeplitzCtSizes[];
 extern const float kToeplitzCtCost[];
 extern const int kToeplitzCtCount;
 
-// FFT length for an n x n Toeplitz product: any even length >= 2n - 1 gives
-// the same result.  Among the compile-time plan sizes >= 2n - 1 within 10 %
-// of the smallest even 2,3,5,7-smooth length, take the one with the least
-// size x relative cost per point; with none, that smooth length.
-int toeplitz_fft_length(int n) {
+// Compile-time plan sizes eligible for an n x n Toeplitz product: >= 2n - 1
+// and within 10 % of the smallest even 2,3,5,7-smooth length, ordered by
+// size x measured relative cost per point (cheapest first).
+static std::vector<int> toeplitz_ct_candidates(int n) {
   const int m = 2 * n - 1;
   const int smooth = fft_next_smooth(m, true);
-  int best = 0;
-  float best_cost = 0.f;
+  std::vector<std::pair<float, int>> c;
   for (int i = 0; i < kToeplitzCtCount; ++i) {
-    const int c = kToeplitzCtSizes[i];
-    if (c < m || c > smooth + smooth / 10) continue;
-    const float cost = (float)c * kToeplitzCtCost[i];
-    if (!best || cost < best_cost) {
-      best = c;
-      best_cost = cost;
+    const int L = kToeplitzCtSizes[i];
+    if (L < m || L > smooth + smooth / 10) continue;
+    c.emplace_back((float)L * kToeplitzCtCost[i], L);
+  }
+  std::stable_sort(c.begin(), c.end(),
+                   [](const std::pair<float, int>& a,
+                      const std::pair<float, int>& b) {
+                     return a.first < b.first;
+                   });
+  std::vector<int> out;
+  for (auto& x : c) out.push_back(x.second);
+  return out;
+}
+
+// FFT lengths picked by Psf::tune_length, keyed by (device, n)
+static std::mutex g_len_mu;
+static std::map<std::pair<int, int>, int> g_len_tuned;
+
+static std::pair<int, int> len_key(int n) {
+  int dev = 0;
+  TC_HIP(hipGetDevice(&dev));
+  return std::make_pair(dev, n);
+}
+
+// FFT length for an n x n Toeplitz product: any even length >= 2n - 1 gives
+// the same result.  The autotuner's choice for this width if there is one;
+// else among the eligible compile-time plan sizes the one with the least
+// size x relative cost per point; with none, the smallest even smooth
+// length.
+int toeplitz_fft_length(int n) {
+  {
+    // (no device query while nothing is tuned: the rule works without one)
+    std::lock_guard<std::mutex> lk(g_len_mu);
+    if (!g_len_tuned.empty()) {
+      auto it = g_len_tuned.find(len_key(n));
+      if (it != g_len_tuned.end()) return it->second;
     }
   }
-  return best ? best : smooth;
+  const std::vector<int> c = toeplitz_ct_candidates(n);
+  return c.empty() ? fft_next_smooth(2 * n - 1, true) : c[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -72,28 +105,96 @@ __global__ void psf_extract_kernel(const float2* __restrict__ c, float* spec,
 
 Psf::Psf(const float* d_psf, int n, hipStream_t stream) : n_(n) {
   TC_REQUIRE(n >= 2, "PSF: image width must be >= 2");
-  M_ = toeplitz_fft_length(n);
-  H_ = M_ / 2 + 1;
   rp_ = n + (n & 1);  // rows of the workspace: whole pairs
   const size_t n1 = 2 * (size_t)n - 1;
   arr_.alloc(sizeof(float) * n1 * n1);
   TC_HIP(hipMemcpyAsync(arr_.as<void>(), d_psf, sizeof(float) * n1 * n1,
                         hipMemcpyDeviceToDevice, stream));
+  build_spectrum(toeplitz_fft_length(n), stream);
+}
+
+void Psf::build_spectrum(int M, hipStream_t stream) {
+  M_ = M;
+  H_ = M_ / 2 + 1;
+  // everything sized by the FFT length starts over
+  spec16_.release();
+  spec16_exp_ = 0;
+  cinv_.release();
+  cinv_slices_ = 0;
+  work_.release();
+  work_slices_ = 0;
   DevBuf c(sizeof(float2) * (size_t)M_ * M_);
   long tot = (long)M_ * M_;
   hipLaunchKernelGGL(psf_place_kernel, dim3(cdiv(tot, 256)), dim3(256), 0,
-                     stream, d_psf, c.as<float2>(), n, M_);
+                     stream, arr_.as<float>(), c.as<float2>(), n_, M_);
   TC_HIP(hipGetLastError());
   // the spectrum keeps columns k1 < H only (real, even PSF)
   fft2d_pruned(c.as<float2>(), 1, M_, H_, 0, -1, true, stream);
   spec_.alloc(sizeof(float) * (size_t)M_ * H_);
-  const double scale = 1.0 / ((double)n * n * n * (double)M_ * M_);
+  const double scale = 1.0 / ((double)n_ * n_ * n_ * (double)M_ * M_);
   long tot2 = (long)M_ * H_;
   hipLaunchKernelGGL(psf_extract_kernel, dim3(cdiv(tot2, 256)), dim3(256), 0,
                      stream, c.as<float2>(), spec_.as<float>(), M_, H_,
                      (float)scale);
   TC_HIP(hipGetLastError());
   TC_HIP(hipStreamSynchronize(stream));
+}
+
+int Psf::tune_length(int nslices, hipStream_t stream) {
+  const std::vector<int> cands = toeplitz_ct_candidates(n_);
+  if (cands.size() < 2) return 0;
+  {
+    std::lock_guard<std::mutex> lk(g_len_mu);
+    auto it = g_len_tuned.find(len_key(n_));
+    if (it != g_len_tuned.end()) {
+      if (it->second != M_) build_spectrum(it->second, stream);
+      return -1;
+    }
+  }
+  // one application of each candidate length on scratch data (one stream,
+  // the 144 MB rule's groups), best of two timed runs after a warm-up; the
+  // static rule's choice (first) is replaced only by one >= 2 % faster
+  constexpr float kMargin = 0.02f;
+  const int S = std::max(1, nslices);
+  const long tot = (long)n_ * n_ * S;
+  DevBuf x(sizeof(float) * (size_t)tot), out(sizeof(float) * (size_t)tot);
+  TC_HIP(hipMemsetAsync(x.as<void>(), 0, sizeof(float) * (size_t)tot, stream));
+  hipEvent_t e0, e1;
+  TC_HIP(hipEventCreate(&e0));
+  TC_HIP(hipEventCreate(&e1));
+  int best = 0;
+  float best_ms = 0.f;
+  try {
+    for (int L : cands) {
+      if (L != M_) build_spectrum(L, stream);
+      float ms = 0.f;
+      for (int rep = 0; rep < 3; ++rep) {
+        TC_HIP(hipEventRecord(e0, stream));
+        apply(x.as<float>(), nullptr, out.as<float>(), S, 0, nullptr, stream);
+        TC_HIP(hipEventRecord(e1, stream));
+        TC_HIP(hipEventSynchronize(e1));
+        float t = 0.f;
+        TC_HIP(hipEventElapsedTime(&t, e0, e1));
+        if (rep == 1 || (rep > 1 && t < ms)) ms = t;
+      }
+      if (best == 0 || ms < (1.0f - kMargin) * best_ms) {
+        best = L;
+        best_ms = ms;
+      }
+    }
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  {
+    std::lock_guard<std::mutex> lk(g_len_mu);
+    g_len_tuned[len_key(n_)] = best;
+  }
+  if (best != M_) build_spectrum(best, stream);
+  return best;
 }
 
 void Psf::reserve(int nslices) {

@@ -116,6 +116,17 @@ class PointSpreadFunction:
                   C.byref(g), _stream())
         return g.value
 
+    def tune_length(self, nslices: int = 1) -> int:
+        """Times the eligible compile-time FFT lengths (where two plans fit
+        this width) on ``nslices`` slices, rebuilds the spectrum at the
+        faster one and keeps it for this width for the process; returns the
+        handle's FFT length."""
+        m = C.c_int()
+        _lib.call("tc_psf_tune_length", self._h, int(nslices), C.byref(m),
+                  _stream())
+        self.fft_size = m.value
+        return m.value
+
     def convolve_mixed(self, x: torch.Tensor, with_dot: bool = False):
         """T x with x rounded to FP16 under a per-slice power-of-two scale and
         FP16 intermediates (the operator of the mixed-precision MBIR form);
