@@ -6,6 +6,7 @@ import os
 import sys
 
 import numpy as np
+import pytest
 
 from conftest import ROOT
 
@@ -21,12 +22,13 @@ def test_mbir2_1024_golden_input_reproducible():
     assert gold["sampled"].shape == (64, 64)
 
 
-def test_mbir2_2048_100it_fixture_fields():
-    """The 100-iteration 2048^2 fixture (the benchmark's FP64 accuracy
-    check) comes from the same input as the 30-iteration one and carries
-    the fields the benchmark reads."""
-    a = np.load(G.fixture_path(2048))
-    b = np.load(G.fixture_path(2048, 100))
+@pytest.mark.parametrize("N", [1024, 2048])
+def test_mbir2_100it_fixture_fields(N):
+    """The 100-iteration fixtures (the workloads' length; the 2048^2 one is
+    the benchmark's FP64 accuracy check) come from the same inputs as the
+    30-iteration ones and carry the fields the benchmark reads."""
+    a = np.load(G.fixture_path(N))
+    b = np.load(G.fixture_path(N, 100))
     assert int(b["iterations"]) == 100 and b["sampled"].shape == (64, 64)
     np.testing.assert_allclose(a["sino_checksum"], b["sino_checksum"],
                                rtol=1e-12)

@@ -1,6 +1,6 @@
 """MBIR at the workload shapes, one slice each (1024^2 / 720 angles,
 padded 1454, FFT length 2916; 2048^2 / 1501 angles, padded 2908, FFT length
-5832), 30 iterations (and 100 at 2048^2), sigma = 100, against the FP64
+5832), 30 and 100 iterations, sigma = 100, against the FP64
 oracle through committed fixtures: the oracle's PSF at these sizes takes minutes, so
 tests/golden/make_mbir2_golden.py ran it once and stored the result sampled
 on a 64 x 64 pixel grid plus whole-image sums.  Tolerance: relative 1e-4 (FP32
@@ -36,15 +36,15 @@ def _case(N):
     return _cases[N]
 
 
-def _gold100():
-    path = G.fixture_path(2048, 100)
+def _gold100(N):
+    path = G.fixture_path(N, 100)
     if not os.path.exists(path):
         pytest.skip(f"fixture {path} not generated")
     return np.load(path)
 
 
 @pytest.mark.parametrize("N,iters", [(1024, G.ITERS), (2048, G.ITERS),
-                                     (2048, 100)])
+                                     (1024, 100), (2048, 100)])
 @pytest.mark.parametrize("precision", ["fp32", "mixed"])
 def test_mbir2_slice_vs_oracle(torch_cuda, N, iters, precision):
     """The 100-iteration case is the benchmark's workload length; its mixed
@@ -54,7 +54,7 @@ def test_mbir2_slice_vs_oracle(torch_cuda, N, iters, precision):
     from tomocam_amd import gpu
     ang, sino, gold = _case(N)
     if iters != G.ITERS:
-        gold = _gold100()
+        gold = _gold100(N)
     announce(f"mbir2 {N}^2 slice, {iters} iterations, vs oracle fixture "
              f"({precision})")
     kw = dict(lipschitz_iters=20) if (precision == "mixed" and
