@@ -657,9 +657,20 @@ void PolarNufft::build_work(hipStream_t stream) {
   work_ready_ = true;
 }
 
+// Slices per fine-grid batch: about 1 GB of fine grid, in whole groups of
+// SP_SB slices (a spreading workgroup accumulates SP_SB slices; a partial
+// group idles part of its accumulator and repeats the tile's segment
+// enumeration and point set-up for fewer slices), rounding up while that
+// stays within 1.25 GB.
 static int fine_batch(int nf, int S) {
-  const double per = 8.0 * nf * (double)nf;
-  int b = (int)std::max(1.0, std::floor(1.0e9 / per));
+  const double per = 8.0 * nf * (double)nf, budget = 1.0e9;
+  const int b0 = (int)std::max(1.0, std::floor(budget / per));
+  int b = b0;
+  const int up = (b0 + SP_SB - 1) / SP_SB * SP_SB;
+  if (up * per <= 1.25 * budget)
+    b = up;
+  else if (b0 >= SP_SB)
+    b = b0 / SP_SB * SP_SB;
   return std::max(1, std::min(S, b));
 }
 
