@@ -64,3 +64,23 @@ def test_tv_hessian_value_matches_oracle():
     v = C.c_float()
     _lib.call("tc_add_tv_hessian", None, 0, 0.1, C.byref(v))
     assert abs(v.value - O.tv_hessian_zero(0.1)) <= 1e-6 * v.value
+
+
+def test_fp64_shape_checks_without_gpu():
+    """The double-precision entry points reject bad shapes before touching
+    the device (host buffers stand in for device pointers: never used)."""
+    import ctypes as C
+    from tomocam_amd import _lib
+    buf = (C.c_double * 8)()
+    p = C.cast(buf, C.c_void_p)
+    out = C.c_double()
+    with pytest.raises(ValueError, match="FP64 gradient: bad shape"):
+        _lib.call("tc_gradient_f64", p, p, p, 0, 16, p, 4, None)
+    with pytest.raises(ValueError, match="FP64 function_value: bad shape"):
+        _lib.call("tc_function_value_f64", p, p, 1, 1, p, 4, C.byref(out),
+                  None)
+    with pytest.raises(ValueError, match="xerror: need at least one"):
+        _lib.call("tc_xerror_f64", p, p, 0, C.byref(out), None)
+    with pytest.raises(ValueError, match="FP64 PSF: bad geometry"):
+        h = C.c_void_p()
+        _lib.call("tc_psf_create_f64", C.byref(h), p, 0, 16, None)

@@ -53,6 +53,8 @@ int tc_gradient_f64(const double* d_x, const double* d_sinoT, double* d_g,
                     void* stream) {
   return tc::guarded([&] {
     TC_REQUIRE(d_x && d_sinoT && d_g && h_angles, "null argument");
+    TC_REQUIRE(nslices >= 1 && npix >= 2 && nproj >= 1,
+               "FP64 gradient: bad shape");
     tc::f64::gradient_nufft(d_x, d_sinoT, d_g, nslices, npix, h_angles, nproj,
                             as_stream(stream));
   });
@@ -63,6 +65,8 @@ int tc_function_value_f64(const double* d_x, const double* d_sino,
                           int nproj, double* h_out, void* stream) {
   return tc::guarded([&] {
     TC_REQUIRE(d_x && d_sino && h_angles && h_out, "null argument");
+    TC_REQUIRE(nslices >= 1 && npix >= 2 && nproj >= 1,
+               "FP64 function_value: bad shape");
     hipStream_t s = as_stream(stream);
     const long n = (long)nslices * nproj * npix;
     tc::DevBuf proj(sizeof(double) * n);
@@ -80,6 +84,7 @@ int tc_psf_create_f64(tc_psf64* out, const double* h_angles, int nproj,
                       int npix, void* stream) {
   return tc::guarded([&] {
     TC_REQUIRE(out && h_angles, "null argument");
+    TC_REQUIRE(nproj >= 1 && npix >= 2, "FP64 PSF: bad geometry");
     auto h = std::make_unique<tc_psf64_s>();
     h->psf = std::make_unique<tc::f64::Psf>(h_angles, nproj, npix,
                                              as_stream(stream));
@@ -113,6 +118,7 @@ int tc_function_value2_f64(tc_psf64 p, const double* d_x,
                            double* h_out, void* stream) {
   return tc::guarded([&] {
     TC_REQUIRE(d_x && d_sinoT && h_out, "null argument");
+    TC_REQUIRE(nslices >= 1, "FP64 function_value2: need at least one slice");
     tc::f64::Psf& P = get_psf64(p);
     hipStream_t s = as_stream(stream);
     const long n = (long)nslices * P.n() * P.n();
@@ -194,6 +200,7 @@ int tc_xerror_f64(const double* d_a, const double* d_b, long n, double* h_out,
                   void* stream) {
   return tc::guarded([&] {
     TC_REQUIRE(d_a && d_b && h_out, "null argument");
+    TC_REQUIRE(n >= 1, "xerror: need at least one element");
     *h_out = tc::f64::diff_norm2(d_a, d_b, n, as_stream(stream));
   });
 }
@@ -213,6 +220,7 @@ int tc_postproc_f64(const double* d_img, double* d_out, int nslices, int n,
                     int npixels, void* stream) {
   return tc::guarded([&] {
     TC_REQUIRE(d_img && d_out, "null argument");
+    TC_REQUIRE(nslices >= 1, "postproc: bad shape");
     tc::f64::postproc(d_img, d_out, nslices, n, npixels, as_stream(stream));
   });
 }

@@ -741,6 +741,7 @@ void backproject(const double* sino, double* img, int S, int P, int N,
 
 void gradient_nufft(const double* x, const double* b, double* g, int S, int N,
                     const double* h_angles, int P, hipStream_t s) {
+  TC_REQUIRE(S >= 1 && N >= 2 && P >= 1, "FP64 gradient: bad shape");
   auto nu = nufft_plan(h_angles, P, N, N);
   const long ns = (long)S * P * N;
   DevBuf c(sizeof(double2) * ns);
