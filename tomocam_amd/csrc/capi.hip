@@ -54,6 +54,9 @@ int tc_project(const float* d_img, float* d_sino, int nslices, int npix,
                const float* h_angles, int nproj, void* stream) {
   return guarded([&] {
     TC_REQUIRE(d_img && d_sino && h_angles, "null argument");
+    // temporaries from the thread's arena (the call synchronises its
+    // stream before they are released): no hipMalloc / hipFree per call
+    tc::ArenaScope arena;
     tc::project(d_img, d_sino, nslices, npix, h_angles, nproj,
                 as_stream(stream));
   });
@@ -63,6 +66,7 @@ int tc_backproject(const float* d_sino, float* d_img, int nslices, int nproj,
                    int npix, const float* h_angles, int fbp, void* stream) {
   return guarded([&] {
     TC_REQUIRE(d_img && d_sino && h_angles, "null argument");
+    tc::ArenaScope arena;
     tc::backproject(d_sino, d_img, nslices, nproj, npix, h_angles, fbp != 0,
                     as_stream(stream));
   });
@@ -74,6 +78,7 @@ int tc_gradient(const float* d_x, const float* d_sinoT, float* d_g,
   return guarded([&] {
     TC_REQUIRE(d_x && d_sinoT && d_g && h_angles, "null argument");
     TC_REQUIRE(nslices >= 1 && npix >= 2 && nproj >= 1, "gradient: bad shape");
+    tc::ArenaScope arena;
     hipStream_t s = as_stream(stream);
     tc::PolarNufft& nu = tc::nufft_plan(h_angles, nproj, npix, npix);
     tc::DevBuf c(sizeof(float2) * (size_t)nslices * nproj * npix);
@@ -91,6 +96,7 @@ int tc_function_value(const float* d_x, const float* d_sino, int nslices,
                       double* h_out, void* stream) {
   return guarded([&] {
     TC_REQUIRE(d_x && d_sino && h_angles && h_out, "null argument");
+    tc::ArenaScope arena;
     hipStream_t s = as_stream(stream);
     const long ns = (long)nslices * nproj * npix;
     tc::DevBuf px(sizeof(float) * ns);
