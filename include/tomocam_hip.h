@@ -232,6 +232,11 @@ int tc_qggmrf_grad_free(const float* d_x, float* d_g, int nslices, int nrows,
                         int ncols, float sigma, float p, void* stream);
 int tc_qggmrf_value_free(const float* d_x, int nslices, int nrows, int ncols,
                          float sigma, float p, double* h_value, void* stream);
+/* The same value into the device double *d_value, asynchronously on
+ * `stream` (for callers that keep the loss on the device). */
+int tc_qggmrf_value_free_dev(const float* d_x, int nslices, int nrows,
+                             int ncols, float sigma, float p, double* d_value,
+                             void* stream);
 
 /* ---------------------------------------------------------------------- */
 /* MBIR iteration                                                           */

@@ -48,6 +48,8 @@ SIGNATURES = {
                             C.c_float, vp],
     "tc_qggmrf_value_free": [vp, C.c_int, C.c_int, C.c_int, C.c_float,
                              C.c_float, c_double_p, vp],
+    "tc_qggmrf_value_free_dev": [vp, C.c_int, C.c_int, C.c_int, C.c_float,
+                                 C.c_float, vp, vp],
     "tc_run2": [vp, vp, vp, vp, C.c_int, C.c_int, C.c_double, C.c_float,
                 C.c_float, C.c_double, C.c_int, C.c_int, c_double_p,
                 c_double_p, c_int_p, vp],

@@ -71,8 +71,8 @@ class QGGMRFFunction(torch.autograd.Function):
         xw = _work(x, x.dtype)
         ctx.save_for_backward(xw)
         ctx.sigma, ctx.p, ctx.dtype = float(sigma), float(p), x.dtype
-        v = gpu.qggmrf_value_free(xw, ctx.sigma, ctx.p)
-        return torch.tensor(v, dtype=x.dtype, device=x.device)
+        # the value stays on the device: no host synchronisation per step
+        return gpu.qggmrf_value_free_tensor(xw, ctx.sigma, ctx.p).to(x.dtype)
 
     @staticmethod
     def backward(ctx, grad: torch.Tensor):

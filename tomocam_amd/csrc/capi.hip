@@ -54,9 +54,9 @@ int tc_project(const float* d_img, float* d_sino, int nslices, int npix,
                const float* h_angles, int nproj, void* stream) {
   return guarded([&] {
     TC_REQUIRE(d_img && d_sino && h_angles, "null argument");
-    // temporaries from the thread's arena (the call synchronises its
-    // stream before they are released): no hipMalloc / hipFree per call
-    tc::ArenaScope arena;
+    // temporaries from the thread's arena, reused in stream order: no
+    // hipMalloc / hipFree and no synchronisation per call
+    tc::ArenaScope arena(as_stream(stream));
     tc::project(d_img, d_sino, nslices, npix, h_angles, nproj,
                 as_stream(stream));
   });
@@ -66,7 +66,7 @@ int tc_backproject(const float* d_sino, float* d_img, int nslices, int nproj,
                    int npix, const float* h_angles, int fbp, void* stream) {
   return guarded([&] {
     TC_REQUIRE(d_img && d_sino && h_angles, "null argument");
-    tc::ArenaScope arena;
+    tc::ArenaScope arena(as_stream(stream));
     tc::backproject(d_sino, d_img, nslices, nproj, npix, h_angles, fbp != 0,
                     as_stream(stream));
   });
@@ -78,8 +78,8 @@ int tc_gradient(const float* d_x, const float* d_sinoT, float* d_g,
   return guarded([&] {
     TC_REQUIRE(d_x && d_sinoT && d_g && h_angles, "null argument");
     TC_REQUIRE(nslices >= 1 && npix >= 2 && nproj >= 1, "gradient: bad shape");
-    tc::ArenaScope arena;
     hipStream_t s = as_stream(stream);
+    tc::ArenaScope arena(s);
     tc::PolarNufft& nu = tc::nufft_plan(h_angles, nproj, npix, npix);
     tc::DevBuf c(sizeof(float2) * (size_t)nslices * nproj * npix);
     nu.type2_real(d_x, c.as<float2>(), nslices, s);
@@ -87,7 +87,6 @@ int tc_gradient(const float* d_x, const float* d_sinoT, float* d_g,
     nu.type1_real(c.as<float2>(), d_g, nslices, (float)(1.0 / n3), s);
     const long N = (long)nslices * npix * npix;
     tc::affine_sub(d_g, d_sinoT, d_g, N, s);
-    TC_HIP(hipStreamSynchronize(s));
   });
 }
 
@@ -403,8 +402,21 @@ int tc_qggmrf_value_free(const float* d_x, int nslices, int nrows, int ncols,
                          float sigma, float p, double* h_value, void* stream) {
   return guarded([&] {
     TC_REQUIRE(d_x && h_value, "null argument");
+    tc::ArenaScope arena;  // synchronises before its temporaries return
     *h_value = tc::qggmrf_value_free(d_x, nslices, nrows, ncols, sigma, p,
                                      as_stream(stream));
+  });
+}
+
+int tc_qggmrf_value_free_dev(const float* d_x, int nslices, int nrows,
+                             int ncols, float sigma, float p, double* d_value,
+                             void* stream) {
+  return guarded([&] {
+    TC_REQUIRE(d_x && d_value, "null argument");
+    TC_REQUIRE(sigma > 0.f, "qGGMRF: sigma must be > 0");
+    tc::ArenaScope arena(as_stream(stream));
+    tc::qggmrf_value_free_dev(d_x, nslices, nrows, ncols, sigma, p, d_value,
+                              as_stream(stream));
   });
 }
 

@@ -26,6 +26,9 @@ float tv_hessian_zero(float sigma);
 // pairs leaving the volume are dropped instead of clamped.
 void add_qggmrf_grad_free(const float* x, float* g, int S, int R, int C,
                           float sigma, float p, hipStream_t stream);
+// R(x) into the device double d_out, asynchronously on `stream`.
+void qggmrf_value_free_dev(const float* x, int S, int R, int C, float sigma,
+                           float p, double* d_out, hipStream_t stream);
 double qggmrf_value_free(const float* x, int S, int R, int C, float sigma,
                          float p, hipStream_t stream);
 

@@ -497,25 +497,30 @@ void add_qggmrf_grad_free(const float* x, float* g, int S, int R, int C,
   TC_HIP(hipGetLastError());
 }
 
-double qggmrf_value_free(const float* x, int S, int R, int C, float sigma,
-                         float p, hipStream_t stream) {
+void qggmrf_value_free_dev(const float* x, int S, int R, int C, float sigma,
+                           float p, double* d_out, hipStream_t stream) {
   TC_REQUIRE(S >= 1 && R >= 1 && C >= 1, "qGGMRF: empty volume");
   const long nvox = (long)S * R * C;
   const unsigned nblk = (unsigned)cdiv(nvox, 256);
-  DevBuf parts(sizeof(double) * (nblk + 1));
+  DevBuf parts(sizeof(double) * nblk);
   hipLaunchKernelGGL(qggmrf_free_kernel<1>, dim3(nblk), dim3(256), 0, stream,
                      x, nullptr, parts.as<double>(), S, R, C,
                      make_rho(sigma, p));
   TC_HIP(hipGetLastError());
   hipLaunchKernelGGL(reduce_strided_kernel, dim3(1), dim3(1024), 0, stream,
-                     parts.as<double>(), (long)nblk, 1,
-                     parts.as<double>() + nblk);
+                     parts.as<double>(), (long)nblk, 1, d_out);
   TC_HIP(hipGetLastError());
-  double out = 0.0;
-  TC_HIP(hipMemcpyAsync(&out, parts.as<double>() + nblk, sizeof(double),
+}
+
+double qggmrf_value_free(const float* x, int S, int R, int C, float sigma,
+                         float p, hipStream_t stream) {
+  DevBuf out(sizeof(double));
+  qggmrf_value_free_dev(x, S, R, C, sigma, p, out.as<double>(), stream);
+  double v = 0.0;
+  TC_HIP(hipMemcpyAsync(&v, out.as<void>(), sizeof(double),
                         hipMemcpyDeviceToHost, stream));
   TC_HIP(hipStreamSynchronize(stream));
-  return out;
+  return v;
 }
 
 float tv_hessian_zero(float sigma) {

@@ -728,8 +728,8 @@ void PolarNufft::type1_real(const float2* c, float* out, int S, float scale,
                        nm_, nf_, nb, scale);
     TC_HIP(hipGetLastError());
   }
-  // the scratch slots are released on return
-  TC_HIP(hipStreamSynchronize(stream));
+  // the scratch slots are released on return: to the thread's arena
+  // (reused in stream order) or by hipFree, which waits for the device
 }
 
 void PolarNufft::type2_real(const float* f, float2* c, int S,
@@ -966,7 +966,6 @@ void project(const float* img, float* sino, int S, int N, const float* h_angles,
   nu.type2_real(img, c.as<float2>(), S, stream);
   rd.apply(c.as<float2>(), false, sino, true, (long)S * P, false,
            1.0f / ((float)N * (float)N), stream);
-  TC_HIP(hipStreamSynchronize(stream));
 }
 
 void backproject(const float* sino, float* img, int S, int P, int N,
@@ -977,7 +976,6 @@ void backproject(const float* sino, float* img, int S, int P, int N,
   DevBuf c(sizeof(float2) * (size_t)S * P * N);
   rd.apply(sino, true, c.as<float2>(), false, (long)S * P, fbp, 1.0f, stream);
   nu.type1_real(c.as<float2>(), img, S, 1.0f / ((float)N * (float)N), stream);
-  TC_HIP(hipStreamSynchronize(stream));
 }
 
 __global__ void fill_ones_kernel(float2* c, long n) {

@@ -25,6 +25,10 @@ struct Arena {
   size_t free_bytes = 0;
   int dev = -1;
   double* pinned = nullptr;  // arena_pinned_dots
+  // the last asynchronous scope's stream and its completion event
+  hipStream_t last = nullptr;
+  hipEvent_t done = nullptr;
+  bool pending = false;
 };
 
 static std::mutex g_pool_mu;
@@ -45,7 +49,12 @@ struct ArenaOwner {
 };
 static thread_local ArenaOwner t_arena;
 
-ArenaScope::ArenaScope() {
+ArenaScope::ArenaScope() { enter(false, nullptr); }
+ArenaScope::ArenaScope(hipStream_t stream) { enter(true, stream); }
+
+void ArenaScope::enter(bool async, hipStream_t stream) {
+  async_ = async;
+  stream_ = stream;
   if (!t_arena.a) {
     int dev = 0;
     TC_HIP(hipGetDevice(&dev));
@@ -61,10 +70,34 @@ ArenaScope::ArenaScope() {
       g_all.push_back(t_arena.a);
     }
   }
+  // blocks released by an asynchronous scope may still be in use on its
+  // stream: wait unless this scope runs on that same stream
+  Arena* a = t_arena.a;
+  if (a->pending && (!async || stream != a->last)) {
+    TC_HIP(hipEventSynchronize(a->done));
+    a->pending = false;
+  }
   ++t_arena.depth;
 }
 
-ArenaScope::~ArenaScope() { --t_arena.depth; }
+ArenaScope::~ArenaScope() {
+  if (--t_arena.depth != 0 || !async_) return;
+  Arena* a = t_arena.a;
+  if (!a->done &&
+      hipEventCreateWithFlags(&a->done, hipEventDisableTiming) != hipSuccess) {
+    // no event: fall back to waiting for the stream now
+    (void)hipGetLastError();
+    (void)hipStreamSynchronize(stream_);
+    return;
+  }
+  if (hipEventRecord(a->done, stream_) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipStreamSynchronize(stream_);
+    return;
+  }
+  a->last = stream_;
+  a->pending = true;
+}
 
 ArenaPause::ArenaPause() : saved_(t_arena.depth) { t_arena.depth = 0; }
 ArenaPause::~ArenaPause() { t_arena.depth = saved_; }

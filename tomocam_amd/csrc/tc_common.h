@@ -69,17 +69,28 @@ int guarded(F&& f) {
 // to the arena's free list (not to hipFree, which synchronises the whole
 // device) and later requests of a similar size reuse them, so a repeated
 // MBIR call allocates nothing after the first.  An arena is owned by one
-// host thread at a time (reuse is ordered by that thread's stream work);
-// when the thread ends, its arena goes back to a process-wide pool for the
-// next thread, with its blocks still cached.  Scopes belong around work
-// that synchronises its streams before the scope ends.
+// host thread at a time; when the thread ends, its arena goes back to a
+// process-wide pool for the next thread, with its blocks still cached.
+// Reuse is ordered by stream work:
+//   - ArenaScope() belongs around work that synchronises its streams
+//     before the scope ends (the MBIR drivers);
+//   - ArenaScope(stream) around work left pending on `stream` (the
+//     asynchronous operators): the outermost such scope records an event
+//     on `stream` when it ends, and a later scope on another stream (or a
+//     synchronising one) waits for that event before any block is reused.
 struct Arena;
 class ArenaScope {
  public:
   ArenaScope();
+  explicit ArenaScope(hipStream_t stream);
   ~ArenaScope();
   ArenaScope(const ArenaScope&) = delete;
   ArenaScope& operator=(const ArenaScope&) = delete;
+
+ private:
+  void enter(bool async, hipStream_t stream);
+  bool async_ = false;
+  hipStream_t stream_ = nullptr;
 };
 // Turns the calling thread's arena off for its lifetime: long-lived objects
 // (FFT twiddle tables, NUFFT and ray-DFT plans) allocate fresh memory,

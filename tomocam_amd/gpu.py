@@ -228,6 +228,23 @@ def qggmrf_value_free(x: torch.Tensor, sigma: float, p: float = 1.2) -> float:
     return v.value
 
 
+def qggmrf_value_free_tensor(x: torch.Tensor, sigma: float,
+                             p: float = 1.2) -> torch.Tensor:
+    """The free-boundary prior value as a 0-d float64 device tensor,
+    computed without a host synchronisation (float64 input: the FP64
+    kernels, whose value comes through the host)."""
+    x3 = _as3d(x)
+    S, R, Cc = x3.shape
+    if _f64(x3):
+        return torch.tensor(qggmrf_value_free(x3, sigma, p),
+                            dtype=torch.float64, device=x3.device)
+    out = torch.empty((), dtype=torch.float64, device=x3.device)
+    _lib.call("tc_qggmrf_value_free_dev", _dev(x3, "x"), S, R, Cc,
+              float(sigma), float(p), _dev(out, "value", torch.float64),
+              _stream())
+    return out
+
+
 def tv_hessian_zero(sigma: float) -> float:
     """Value gpu::add_tv_hessian would add (it never reaches the host)."""
     v = C.c_float()
