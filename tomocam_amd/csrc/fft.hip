@@ -1,5 +1,6 @@
 // Host plans and the generic batched line FFT kernel.
 #include <cmath>
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -173,8 +174,12 @@ static void launch_lines_nb(const LineFFTArgs& a, const FFTLength& F, int lpw,
 }
 
 void fft_lines(const LineFFTArgs& a, int dir, hipStream_t stream) {
-  const FFTLength& F = fft_length(a.L);
   TC_REQUIRE(a.nin <= a.L && a.nout <= a.L, "fft_lines: nin/nout exceed L");
+  {
+    const char* e = std::getenv("TOMOCAM_HIP_CT_LINES");
+    if (!(e && e[0] == '0') && fft_lines_ct(a, dir, stream)) return;
+  }
+  const FFTLength& F = fft_length(a.L);
   // lines per workgroup: enough for coalescing of column transforms, bounded
   // by 64 KiB of LDS
   int lpw = 1;

@@ -21,8 +21,16 @@ struct LineFFTArgs {
 };
 
 // dir = -1: exp(-2 pi i k n / L) (forward); +1: exp(+2 pi i k n / L); both
-// unnormalised (times `scale`).
+// unnormalised (times `scale`).  Lengths with a compile-time line plan run
+// on it (fft_lines_ct.hip) unless TOMOCAM_HIP_CT_LINES=0; others on the
+// runtime-plan kernel.
 void fft_lines(const LineFFTArgs& a, int dir, hipStream_t stream);
+
+// The compile-time line plans: whether L has one, the smallest planned
+// length >= n (0: none), and the transform (false: no plan for a.L).
+bool fft_lines_ct_has(int L);
+int fft_lines_ct_length(int n);
+bool fft_lines_ct(const LineFFTArgs& a, int dir, hipStream_t stream);
 
 // In-place unnormalised 2D transform of `nbatch` complex [rows][cols] arrays.
 void fft2d(float2* data, int nbatch, int rows, int cols, long row_pitch,

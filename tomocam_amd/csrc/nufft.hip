@@ -561,13 +561,24 @@ __global__ void precorrect_kernel(const float* __restrict__ f, float2* fine,
 // PolarNufft
 // ---------------------------------------------------------------------------
 
+// Fine-grid size for at least m points: the smallest length with a
+// compile-time line FFT plan when it exceeds the smallest even 2,3,5,7-smooth
+// length by at most 1/8 (the planned transforms run several times faster
+// than the runtime-plan kernel, and a larger grid only raises the
+// upsampling factor above 2), else that smooth length.
+static int fine_length(int m) {
+  const int smooth = fft_next_smooth(m, true);
+  const int c = fft_lines_ct_length(m);
+  return (c > 0 && c <= smooth + smooth / 8) ? c : smooth;
+}
+
 PolarNufft::PolarNufft(const float* h_angles, int nproj, int npix, int nmodes)
     : P_(nproj), N_(npix), nm_(nmodes) {
   ArenaPause no_arena;  // cached plan, tables filled synchronously
   TC_REQUIRE(nproj >= 1 && npix >= 2 && nmodes >= 1, "NUFFT: bad geometry");
   TC_REQUIRE(nproj <= SP_LIST, "NUFFT: too many projection angles");
   TC_REQUIRE(npix < 65536, "NUFFT: too many samples per ray");
-  nf_ = fft_next_smooth(std::max(2 * nmodes, 2 * NUFFT_W), true);
+  nf_ = fine_length(std::max(2 * nmodes, 2 * NUFFT_W));
   beta_ = (float)(2.30 * NUFFT_W);
   std::vector<double> hx(P_), hy(P_);
   const double sc = (double)nf_ / (double)(npix - 1);
