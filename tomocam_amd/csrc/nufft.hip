@@ -458,7 +458,7 @@ spread_fixup_kernel(SpreadArgs a, const int4* __restrict__ fix,
 }
 
 // ---------------------------------------------------------------------------
-// interpolation (type 2): one thread per polar point, all slices of a batch
+// interpolation (type 2): eight lanes per polar point, all slices of a batch
 // ---------------------------------------------------------------------------
 
 struct InterpArgs {
@@ -470,49 +470,89 @@ struct InterpArgs {
   float beta;
 };
 
+// Lane b of a point's group of eight reads footprint column b of every row
+// (its eight rows from one slice are eight loads), so a wave's gather for
+// one row covers eight points x 64 contiguous bytes instead of 64 scattered
+// cells.  The lane weights its column sum by wx[b]; the eight columns are
+// then summed across the group for eight slices at once by a transposing
+// butterfly (exchange halves with lanes b^4, b^2, b^1: 14 shuffles per
+// eight slices), after which lane b holds slice b of the chunk and writes
+// it.  The summation order is fixed, so results are deterministic.
+__device__ __forceinline__ float2 shfl_xor2(float2 v, int m) {
+  return make_float2(__shfl_xor(v.x, m, 8), __shfl_xor(v.y, m, 8));
+}
+
 __global__ void __launch_bounds__(256) interp_kernel(InterpArgs a) {
-  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long)a.P * a.N) return;
-  const int p = (int)(idx / a.N);
-  const int j = (int)(idx - (long)p * a.N);
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long pt = gid >> 3;
+  const int b = (int)(gid & 7);
+  // groups of eight are aligned to eight lanes: a group is all in or out
+  if (pt >= (long)a.P * a.N) return;
+  const int p = (int)(pt / a.N);
+  const int j = (int)(pt - (long)p * a.N);
   const double hw = 0.5 * NUFFT_W;
   const double t = (double)(j - a.cidx);
   const double u = 0.5 * a.nf + t * a.ax[p];
   const double v = 0.5 * a.nf + t * a.ay[p];
   const int l0x = (int)ceil(u - hw);
   const int l0y = (int)ceil(v - hw);
-  float wx[NUFFT_W], wy[NUFFT_W];
-  int ix[NUFFT_W], iy[NUFFT_W];
+  const float wxb = es_kernel((float)((l0x + b - u) * (2.0 / NUFFT_W)), a.beta);
+  const float wyb = es_kernel((float)((l0y + b - v) * (2.0 / NUFFT_W)), a.beta);
+  int qx = l0x + b;
+  qx += qx < 0 ? a.nf : (qx >= a.nf ? -a.nf : 0);
+  float wy[NUFFT_W];
+  long roff[NUFFT_W];
 #pragma unroll
-  for (int b = 0; b < NUFFT_W; ++b) {
-    wx[b] = es_kernel((float)((l0x + b - u) * (2.0 / NUFFT_W)), a.beta);
-    wy[b] = es_kernel((float)((l0y + b - v) * (2.0 / NUFFT_W)), a.beta);
-    int qx = l0x + b, qy = l0y + b;
-    qx += qx < 0 ? a.nf : (qx >= a.nf ? -a.nf : 0);
+  for (int r = 0; r < NUFFT_W; ++r) {
+    wy[r] = __shfl(wyb, r, 8);
+    int qy = l0y + r;
     qy += qy < 0 ? a.nf : (qy >= a.nf ? -a.nf : 0);
-    ix[b] = qx;
-    iy[b] = qy;
+    roff[r] = (long)qy * a.nf + qx;
   }
-  for (int s = 0; s < a.nsl; ++s) {
-    const float2* g = a.fine + (long)s * a.nf * a.nf;
-    float2 acc = make_float2(0.f, 0.f);
-    // 4 rows (32 gathers) in flight: fully unrolled, 64 gathers held 174
-    // VGPRs and two waves per SIMD; 4 measured 17 % faster (projection of
-    // 32 x 1024^2) than 8, and faster than 1 or 2
-#pragma unroll 4
-    for (int r = 0; r < NUFFT_W; ++r) {
-      const float2* row = g + (long)iy[r] * a.nf;
-      float2 ra = make_float2(0.f, 0.f);
+  const long plane = (long)a.nf * a.nf;
+  const long cplane = (long)a.P * a.N;
+  for (int s0 = 0; s0 < a.nsl; s0 += 8) {
+    float2 part[8];
+#pragma unroll 2
+    for (int q = 0; q < 8; ++q) {
+      float2 acc = make_float2(0.f, 0.f);
+      if (s0 + q < a.nsl) {
+        const float2* g = a.fine + (long)(s0 + q) * plane;
+        float2 val[NUFFT_W];
 #pragma unroll
-      for (int b = 0; b < NUFFT_W; ++b) {
-        const float2 q = row[ix[b]];
-        ra.x = fmaf(q.x, wx[b], ra.x);
-        ra.y = fmaf(q.y, wx[b], ra.y);
+        for (int r = 0; r < NUFFT_W; ++r) val[r] = g[roff[r]];
+#pragma unroll
+        for (int r = 0; r < NUFFT_W; ++r) {
+          acc.x = fmaf(val[r].x, wy[r], acc.x);
+          acc.y = fmaf(val[r].y, wy[r], acc.y);
+        }
       }
-      acc.x = fmaf(ra.x, wy[r], acc.x);
-      acc.y = fmaf(ra.y, wy[r], acc.y);
+      part[q] = make_float2(acc.x * wxb, acc.y * wxb);
     }
-    a.c[(long)s * a.P * a.N + idx] = acc;
+    // lanes with bit m set keep the upper half of their slice set
+    float2 h4[4], h2[2];
+    const bool u4 = (b & 4) != 0, u2 = (b & 2) != 0, u1 = (b & 1) != 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float2 keep = u4 ? part[4 + k] : part[k];
+      const float2 give = u4 ? part[k] : part[4 + k];
+      const float2 got = shfl_xor2(give, 4);
+      h4[k] = make_float2(keep.x + got.x, keep.y + got.y);
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const float2 keep = u2 ? h4[2 + k] : h4[k];
+      const float2 give = u2 ? h4[k] : h4[2 + k];
+      const float2 got = shfl_xor2(give, 2);
+      h2[k] = make_float2(keep.x + got.x, keep.y + got.y);
+    }
+    const float2 keep = u1 ? h2[1] : h2[0];
+    const float2 give = u1 ? h2[0] : h2[1];
+    const float2 got = shfl_xor2(give, 1);
+    // lane b now holds slice s0 + b (bits 4, 2, 1 of b chose the halves)
+    if (s0 + b < a.nsl)
+      a.c[(long)(s0 + b) * cplane + pt] =
+          make_float2(keep.x + got.x, keep.y + got.y);
   }
 }
 
@@ -768,7 +808,7 @@ void PolarNufft::type2_real(const float* f, float2* c, int S,
     a.nf = nf_;
     a.nsl = nb;
     a.beta = beta_;
-    hipLaunchKernelGGL(interp_kernel, dim3(cdiv((long)P_ * N_, 256)),
+    hipLaunchKernelGGL(interp_kernel, dim3(cdiv(8L * P_ * N_, 256)),
                        dim3(256), 0, stream, a);
     TC_HIP(hipGetLastError());
   }
