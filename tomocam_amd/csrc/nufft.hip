@@ -118,7 +118,13 @@ __device__ __forceinline__ int block_scan_incl(int v, int* wsum) {
   return v + off;
 }
 
-constexpr int SPREAD_CHUNK = 1024;  // points per work item
+// Points per work item: a tile with more points is split into chunks whose
+// partial tiles a fix-up pass adds.  Small grids (fewer than 4096 tiles, a
+// few workgroups per CU) take 512 so the central tiles, where every ray
+// passes, do not leave a tail; larger grids take 1024 (less fix-up).
+// Measured with the longest-first order below: 8 x 512^2 at 360 angles
+// 0.50 -> 0.44 ms with 512; 32 x 1454^2 at 720 angles 8.42 (1024) vs 8.75.
+static int spread_chunk(int ntiles) { return ntiles < 4096 ? 512 : 1024; }
 
 // Enumerates the ray segments (superset j ranges) of shift (U0, V0) whose
 // kernel footprints meet the tile window; leaves seg_pj[k] = p | jlo << 11
@@ -689,15 +695,35 @@ void PolarNufft::build_work(hipStream_t stream) {
   TC_HIP(hipStreamSynchronize(stream));
   std::vector<int4> work, fix;
   int slot = 0;
+  const int chunk = spread_chunk(ntiles);
   for (int t = 0; t < ntiles; ++t) {
     const int c = counts[t];
-    const int nch = std::max(1, (c + SPREAD_CHUNK - 1) / SPREAD_CHUNK);
-    work.push_back(make_int4(t, 0, std::min(c, SPREAD_CHUNK), -1));
+    const int nch = std::max(1, (c + chunk - 1) / chunk);
+    work.push_back(make_int4(t, 0, std::min(c, chunk), -1));
     if (nch > 1) {
       fix.push_back(make_int4(t, slot, nch - 1, 0));
       for (int k = 1; k < nch; ++k)
-        work.push_back(make_int4(t, k * SPREAD_CHUNK,
-                                 std::min(c, (k + 1) * SPREAD_CHUNK), slot++));
+        work.push_back(make_int4(t, k * chunk, std::min(c, (k + 1) * chunk),
+                                 slot++));
+    }
+  }
+  // Heaviest items first: a workgroup's time grows with its points, and
+  // workgroups start in block order, so the long ones must not be last
+  // (16 x 2908^2 at 1501 angles: 28.6 -> 19.9 ms per backprojection).
+  // Block b reads item xcd_remap(b) (each XCD takes a contiguous range of
+  // the list), so rank b of the sorted order goes to index xcd_remap(b).
+  // Each item writes its own tile or slot, and the fix-up adds slots in
+  // slot order, so the result does not depend on the order.
+  {
+    std::vector<int4> byw(work);
+    std::stable_sort(byw.begin(), byw.end(), [](const int4& x, const int4& y) {
+      return x.z - x.y > y.z - y.y;
+    });
+    const int n = (int)byw.size();
+    const int q = n >> 3, rmd = n & 7;
+    for (int b = 0; b < n; ++b) {
+      const int x = b & 7, t = b >> 3;
+      work[x * q + (x < rmd ? x : rmd) + t] = byw[b];
     }
   }
   nwork_ = (int)work.size();
