@@ -445,21 +445,38 @@ spread_fixup_kernel(SpreadArgs a, const int4* __restrict__ fix,
   const int ty = f.x / a.ntx, tx = f.x - (f.x / a.ntx) * a.ntx;
   const int X0 = tx * SP_T, Y0 = ty * SP_T;
   const int X1 = min(X0 + SP_T, a.nf), Y1 = min(Y0 + SP_T, a.nf);
-  for (int i = threadIdx.x; i < ns * SP_T * SP_T; i += 256) {
+  // each thread sums its 16 elements slot by slot, so the 16 loads of a
+  // slot are in flight together (one element at a time left each load's
+  // latency exposed); the per-element order is still slot order
+  constexpr int PER = SP_SB * SP_T * SP_T / 256;
+  const int nel = ns * SP_T * SP_T;
+  float2 sum[PER];
+#pragma unroll
+  for (int e = 0; e < PER; ++e) sum[e] = make_float2(0.f, 0.f);
+  for (int k = 0; k < f.z; ++k) {
+    const float2* src = scratch + ((long)(f.y + k) * gridDim.y + blockIdx.y) *
+                                      (SP_SB * SP_T * SP_T);
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int i = threadIdx.x + 256 * e;
+      if (i < nel) {
+        const float2 q = src[i];
+        sum[e].x += q.x;
+        sum[e].y += q.y;
+      }
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    const int i = threadIdx.x + 256 * e;
     const int s = i / (SP_T * SP_T);
     const int cell = i - s * SP_T * SP_T;
     const int ly = cell / SP_T, lx = cell - ly * SP_T;
-    if (Y0 + ly >= Y1 || X0 + lx >= X1) continue;
-    float2 sum = make_float2(0.f, 0.f);
-    for (int k = 0; k < f.z; ++k) {
-      const float2 q = scratch[((long)(f.y + k) * gridDim.y + blockIdx.y) *
-                                   (SP_SB * SP_T * SP_T) + i];
-      sum.x += q.x;
-      sum.y += q.y;
+    if (i < nel && Y0 + ly < Y1 && X0 + lx < X1) {
+      float2* o = a.fine + ((long)(s0 + s) * a.nf + Y0 + ly) * a.nf + X0 + lx;
+      const float2 cur = *o;
+      *o = make_float2(cur.x + sum[e].x, cur.y + sum[e].y);
     }
-    float2* o = a.fine + ((long)(s0 + s) * a.nf + Y0 + ly) * a.nf + X0 + lx;
-    float2 cur = *o;
-    *o = make_float2(cur.x + sum.x, cur.y + sum.y);
   }
 }
 
