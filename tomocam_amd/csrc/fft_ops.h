@@ -31,6 +31,12 @@ void fft_lines(const LineFFTArgs& a, int dir, hipStream_t stream);
 bool fft_lines_ct_has(int L);
 int fft_lines_ct_length(int n);
 bool fft_lines_ct(const LineFFTArgs& a, int dir, hipStream_t stream);
+// Chirp-z DFT of `nrays` rays of N samples on planned length L >= 2N - 1
+// (false: no plan for L): out[j] = chirp[j] IFFT_L(FFT_L(in chirp) bhat)[j]
+// times `scale` (and |j - N/2| / N if `ramp`).
+bool bluestein_ct(const void* in, bool in_real, void* out, bool out_real,
+                  const float2* chirp, const float2* bhat, int N, int L,
+                  long nrays, bool ramp, float scale, hipStream_t stream);
 
 // In-place unnormalised 2D transform of `nbatch` complex [rows][cols] arrays.
 void fft2d(float2* data, int nbatch, int rows, int cols, long row_pitch,

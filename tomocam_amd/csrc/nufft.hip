@@ -2,6 +2,7 @@
 // Bluestein ray DFT and the projectors built on them.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -909,7 +910,14 @@ ray_dft_kernel(const void* __restrict__ in, void* __restrict__ out,
 RayDft::RayDft(int N, int sign) : N_(N) {
   ArenaPause no_arena;  // cached plan, tables filled synchronously
   TC_REQUIRE(N >= 1, "ray DFT: N must be >= 1");
-  L_ = fft_next_smooth(2 * N - 1, false);
+  // the chirp-z length: a planned FFT length when within 1/8 of the
+  // smallest 2,3,5,7-smooth one (the planned kernel runs forward transform,
+  // multiply and inverse in registers)
+  {
+    const int smooth = fft_next_smooth(2 * N - 1, false);
+    const int c = fft_lines_ct_length(2 * N - 1);
+    L_ = (c > 0 && c <= smooth + smooth / 8) ? c : smooth;
+  }
   const int c = N / 2;
   const long twoN = 2L * N;
   std::vector<float2> ch(N), b(L_, make_float2(0.f, 0.f));
@@ -946,6 +954,13 @@ RayDft::RayDft(int N, int sign) : N_(N) {
 void RayDft::apply(const void* in, bool in_real, void* out, bool out_real,
                    long nrays, bool ramp, float scale, hipStream_t stream) {
   if (nrays <= 0) return;
+  {
+    const char* e = std::getenv("TOMOCAM_HIP_CT_LINES");
+    if (!(e && e[0] == '0') &&
+        bluestein_ct(in, in_real, out, out_real, chirp_.as<float2>(),
+                     bhat_.as<float2>(), N_, L_, nrays, ramp, scale, stream))
+      return;
+  }
   const FFTLength& F = fft_length(L_);
   LdsLaunch ll = plan_lds_launch(F, 4, 80 * 1024);
   unsigned nblk = cdiv(nrays, ll.lpw);
