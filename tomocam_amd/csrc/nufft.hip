@@ -446,22 +446,45 @@ spread_fixup_kernel(SpreadArgs a, const int4* __restrict__ fix,
   const int ty = f.x / a.ntx, tx = f.x - (f.x / a.ntx) * a.ntx;
   const int X0 = tx * SP_T, Y0 = ty * SP_T;
   const int X1 = min(X0 + SP_T, a.nf), Y1 = min(Y0 + SP_T, a.nf);
-  // each thread sums its 16 elements slot by slot, so the 16 loads of a
-  // slot are in flight together (one element at a time left each load's
-  // latency exposed); the per-element order is still slot order
-  constexpr int PER = SP_SB * SP_T * SP_T / 256;
+  // blockIdx.z picks a quarter of the SP_SB x SP_T x SP_T elements, so a
+  // tile's slots are summed by four workgroups; each thread sums its four
+  // elements over the slots four slots at a time (16 loads in flight);
+  // the per-element order is slot order
+  constexpr int PART = SP_SB * SP_T * SP_T / 4;
+  constexpr int PER = PART / 256;
   const int nel = ns * SP_T * SP_T;
+  const int e0 = blockIdx.z * PART + threadIdx.x;
+  const long slot_stride = (long)gridDim.y * (SP_SB * SP_T * SP_T);
+  const float2* base = scratch + (long)f.y * slot_stride +
+                       (long)blockIdx.y * (SP_SB * SP_T * SP_T);
   float2 sum[PER];
 #pragma unroll
   for (int e = 0; e < PER; ++e) sum[e] = make_float2(0.f, 0.f);
-  for (int k = 0; k < f.z; ++k) {
-    const float2* src = scratch + ((long)(f.y + k) * gridDim.y + blockIdx.y) *
-                                      (SP_SB * SP_T * SP_T);
+  int k = 0;
+  for (; k + 4 <= f.z; k += 4) {
+    float2 q[4][PER];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int e = 0; e < PER; ++e) {
+        const int i = e0 + 256 * e;
+        q[u][e] = i < nel ? base[(long)(k + u) * slot_stride + i]
+                          : make_float2(0.f, 0.f);
+      }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int e = 0; e < PER; ++e) {
+        sum[e].x += q[u][e].x;
+        sum[e].y += q[u][e].y;
+      }
+  }
+  for (; k < f.z; ++k) {
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
-      const int i = threadIdx.x + 256 * e;
+      const int i = e0 + 256 * e;
       if (i < nel) {
-        const float2 q = src[i];
+        const float2 q = base[(long)k * slot_stride + i];
         sum[e].x += q.x;
         sum[e].y += q.y;
       }
@@ -469,7 +492,7 @@ spread_fixup_kernel(SpreadArgs a, const int4* __restrict__ fix,
   }
 #pragma unroll
   for (int e = 0; e < PER; ++e) {
-    const int i = threadIdx.x + 256 * e;
+    const int i = e0 + 256 * e;
     const int s = i / (SP_T * SP_T);
     const int cell = i - s * SP_T * SP_T;
     const int ly = cell / SP_T, lx = cell - ly * SP_T;
@@ -810,7 +833,7 @@ void PolarNufft::type1_real(const float2* c, float* out, int S, float scale,
                        scratch.as<float2>());
     TC_HIP(hipGetLastError());
     if (nfix_) {
-      hipLaunchKernelGGL(spread_fixup_kernel, dim3((unsigned)nfix_, ng),
+      hipLaunchKernelGGL(spread_fixup_kernel, dim3((unsigned)nfix_, ng, 4),
                          dim3(256), 0, stream, a, fix_.as<int4>(),
                          scratch.as<float2>());
       TC_HIP(hipGetLastError());
