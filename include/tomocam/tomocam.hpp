@@ -504,7 +504,7 @@ DArray<T> mbir2(std::optional<DArray<T>> guess, const DArray<T>& sino,
   else
     detail::check(tc_mbir2(d_sino.get(), d_guess ? d_guess->get() : nullptr,
                            d_out.get(), S, P, N, angles.data(), center,
-                           num_iters, sigma, tol, xtol, 10, 1, nullptr,
+                           num_iters, sigma, tol, xtol, 25, 1, nullptr,
                            nullptr),
                   "mbir2");
   d_out.to_host(out.begin());

@@ -30,7 +30,7 @@ struct Mbir2Options {
   int num_iters = 100;
   float sigma = 100.f;
   float p = 1.2f;
-  int refresh_every = 10;
+  int refresh_every = 25;
   int verbose = 0;
   // mixed precision (FP16 gradient and spectrum; see Run2Options::mixed)
   bool mixed = false;
