@@ -42,11 +42,11 @@ WORKLOADS = {
     "mbir-64x2048": dict(
         slices=64, npix=2048, nproj=1501, iterations=100, sigma=100.0, p=1.2,
         noise=0.005, seed_group=3, lipschitz_iters=20,
-        label="Toeplitz MBIR, mixed precision (FP16 gradient and PSF "
-              "spectrum storage, FP32 FFTs, intermediates and state, FP64 "
-              "line search, L from 20 power-iteration steps) against plain "
-              "FP32: 64 slices 2048x2048, 1501 angles x 2048 rays, "
-              "100 iterations, qGGMRF sigma=100 p=1.2"),
+        label="Toeplitz MBIR, mixed precision (FP16 gradient storage, FP32 "
+              "spectrum, FFTs, intermediates and state, FP64 line search, L "
+              "from 20 power-iteration steps) against plain FP32: 64 slices "
+              "2048x2048, 1501 angles x 2048 rays, 100 iterations, qGGMRF "
+              "sigma=100 p=1.2"),
     "plain-fp32-32x1024": dict(
         slices=32, npix=1024, nproj=720, iterations=100, sigma=0.1, p=1.2,
         noise=0.01, seed_group=1,
@@ -500,9 +500,8 @@ def main():
     cols = "toeplitz_cols_f16" if headline else "toeplitz_cols"
     if cols in kernels and info and info.get("padded"):
         n = int(info["padded"])
-        # mixed: FP32 intermediates, FP16 spectrum
-        valu = valu_roofline(kernels[cols], cols, n, fft_length(n), 8,
-                             2 if headline else 4)
+        # both forms: FP32 intermediates and spectrum
+        valu = valu_roofline(kernels[cols], cols, n, fft_length(n), 8, 4)
 
     fp32 = None
     if headline:
@@ -550,8 +549,7 @@ def main():
         torch.cuda.empty_cache()
         extra = extra_workloads(torch, gpu, phantom)
 
-    mixed_dtype = "f16 storage (gradient, PSF spectrum) / f32 compute / " \
-                  "f64 line search"
+    mixed_dtype = "f16 storage (gradient) / f32 compute / f64 line search"
     out = {
         "metric": "MBIR slices/s (mixed-prec vs plain FP32) at relL2≤1e-4 "
                   "vs FP64; HBM GB/s",

@@ -512,13 +512,13 @@ DArray<T> mbir2(std::optional<DArray<T>> guess, const DArray<T>& sino,
 }
 
 // Mixed-precision form of mbir2 (no counterpart in the reference): FP16
-// gradient and Toeplitz intermediates, FP32 FFTs and state, FP64 dots, T x
+// gradient, FP32 spectrum, intermediates, FFTs and state, FP64 dots, T x
 // refreshed in FP32 every `refresh_every` iterations.  The padded width must
 // have a compile-time FFT plan (std::invalid_argument otherwise).
 template <typename T>
 DArray<T> mbir2_mixed(std::optional<DArray<T>> guess, const DArray<T>& sino,
                       std::vector<T> angles, T center, int num_iters, T sigma,
-                      int refresh_every = 10) {
+                      int refresh_every = 25) {
   detail::require_float<T>();
   const int S = sino.nslices(), P = sino.nrows(), N = sino.ncols();
   detail::Dev<T> d_sino(sino.begin(), sino.size());

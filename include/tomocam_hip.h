@@ -180,8 +180,9 @@ int tc_psf_tune_length(tc_psf p, int nslices, int* fft_size, void* stream);
 
 /* Mixed-precision T x (no counterpart in the reference): x rounded to FP16
  * under a per-slice power-of-two scale, the three passes computing in FP32
- * with FP16 intermediates and FP16 spectrum (the operator the mixed MBIR
- * form applies to its gradient).  Needs an FFT length with compile-time
+ * with FP16 intermediates and FP16 spectrum (the FP16-storage operator; the
+ * mixed MBIR form applies T to its FP16 gradient with FP32 intermediates
+ * and spectrum).  Needs an FFT length with compile-time
  * passes.  Optional h_xTx as for tc_toeplitz_apply, here for the rounded x.
  * Synchronises `stream`. */
 int tc_toeplitz_apply_mixed(tc_psf p, const float* d_x, float* d_out,
@@ -272,8 +273,8 @@ int tc_mbir2(const float* d_sino, const float* d_x0, float* d_recon,
 
 /* Mixed-precision form of tc_mbir2 (no counterpart in the reference, whose
  * Python path is FP32 throughout): the same pipeline and iteration, with
- * the gradient stored in FP16 under a per-slice power-of-two scale and the
- * PSF spectrum in FP16, FFT butterflies, Toeplitz intermediates and NUFFT
+ * the gradient stored in FP16 under a per-slice power-of-two scale, the
+ * PSF spectrum, FFT butterflies, Toeplitz intermediates and NUFFT
  * spreading in FP32, the iterate, T x and b in FP32, line-search dots in
  * FP64, and T sol recomputed in FP32 every refresh_every accepted
  * iterations (0 = never).  The step and T g use the same rounded gradient,

@@ -51,16 +51,15 @@ struct Run2Options {
   float sigma = 1.0f;
   float p = 1.2f;
   double sino_norm = 0.0;  // ||y||^2 of the (normalised, padded) sinogram
-  // recompute T(sol), T(x) every k accepted steps.  FP32: 25 keeps the
-  // drift of the recurrence at FP32 rounding (7.4e-7 against FP64 after 100
-  // iterations at 2048^2, 6.9e-7 with 10) for 3 extra applications per 100
-  // iterations instead of 9 (3.5 % faster).  The mixed form passes 10: its
-  // T g also carries the FP16 spectrum's rounding, which accumulates.
+  // recompute T(sol), T(x) every k accepted steps: 25 keeps the drift of
+  // the recurrence at FP32 rounding (against FP64 after 100 iterations at
+  // 2048^2: 7.4e-7 FP32, 6.48e-6 mixed; 6.9e-7 and 6.46e-6 with 10) for 3
+  // extra applications per 100 iterations instead of 9 (3.5 % faster)
   int refresh_every = 25;
   // Mixed precision: the gradient is stored in FP16 with a per-slice
-  // power-of-two scale, and T g reads it and an FP16 copy of the spectrum
-  // (FP32 arithmetic and FP32 intermediates); the state (sol, x, T sol,
-  // T x, b) stays FP32 and the dots FP64.  The step and T g use the same
+  // power-of-two scale, and T g reads it (FP32 arithmetic, spectrum and
+  // intermediates); the state (sol, x, T sol, T x, b) stays FP32 and the
+  // dots FP64.  The step and T g use the same
   // rounded g, so the T y recurrence drifts only by FP32 rounding, as in
   // the FP32 loop; the refresh above bounds that.
   bool mixed = false;

@@ -401,8 +401,9 @@ const ToeplitzStreams* Psf::streams_for(int nslices, int elem_bytes) {
   return &ts_;
 }
 
+// spec: the FP16 copy with FP16 intermediates (wt16), else the FP32 one
 bool toeplitz_apply_ct_half(int M, const void* x16, float* out, void* wt,
-                            const void* spec16, int n, int H, int rp,
+                            const void* spec, int n, int H, int rp,
                             int nslices, double* partials, hipStream_t stream,
                             double img_b, double spec_b,
                             const Psf::StepCommit* sc,
@@ -439,7 +440,10 @@ bool Psf::run_half(const void* g16, const float* gsc, float* out,
   TC_REQUIRE(g16 && gsc, "Toeplitz: null FP16 input or scales");
   if (toeplitz_ct_lpw(M_) == 0) return false;
   const long nspec = (long)M_ * H_;
-  if (spec16_.bytes() == 0) {
+  // FP32 intermediates read the FP32 spectrum (an FP16 copy measured no
+  // faster there, and its rounding accumulated in the MBIR's T sol
+  // recurrence); FP16 intermediates read the FP16 copy
+  if (wt16 && spec16_.bytes() == 0) {
     // FP16 copy of the spectrum, its largest magnitude mapped into
     // (2^13, 2^14]
     DevBuf mx(sizeof(float));
@@ -471,15 +475,18 @@ bool Psf::run_half(const void* g16, const float* gsc, float* out,
   (void)std::frexp((double)n_, &nb);  // 2^(nb-1) <= n < 2^nb
   ToeplitzHalf h;
   h.gsc = gsc;
-  h.sinv = std::ldexp(1.0f, -spec16_exp_);
+  const int e = wt16 ? spec16_exp_ : 0;  // the spectrum's scale exponent
+  h.sinv = std::ldexp(1.0f, -e);
   h.cinv = cinv_.as<float>();
   h.wsc = std::ldexp(1.0f, -(nb + 1));  // n * 2^16 * wsc < 2^15
-  h.cbase = std::ldexp(1.0f, nb + 1 - spec16_exp_);
-  h.pconst = wt16 ? std::ldexp(1.0f, 2 * (nb + 1) - spec16_exp_) : h.sinv;
+  h.cbase = std::ldexp(1.0f, nb + 1 - e);
+  h.pconst = wt16 ? std::ldexp(1.0f, 2 * (nb + 1) - e) : 1.0f;
   const double img_b = 4.0 * nslices * (double)n_ * n_;
   const double spec_b = 8.0 * nslices * (double)n_ * H_;
+  const void* spec = wt16 ? spec16_.as<void>()
+                          : static_cast<const void*>(spectrum());
   return toeplitz_apply_ct_half(M_, g16, c ? c->Tdst : out, work_.as<void>(),
-                                spec16_.as<void>(), n_, H_, rp_, nslices,
+                                spec, n_, H_, rp_, nslices,
                                 dot_partials, stream, img_b, spec_b, c, h,
                                 wt16, streams_for(nslices, wt16 ? 4 : 8));
 }

@@ -165,9 +165,9 @@ __device__ __forceinline__ float2 unpack_half2(unsigned u) {
 // per-slice power-of-two scale (x = x16 / gsc); HM_WT16 the intermediates
 // Wt are FP16 (pass A output at a fixed scale, pass B output with a scale
 // per 32 threads); HM_SPEC16 the spectrum is read from its FP16 copy.
-// Arithmetic is FP32 throughout.  Forms used: 0 (FP32), HM_IN16 |
-// HM_SPEC16 (the mixed-precision MBIR: FP16 gradient and spectrum, FP32
-// intermediates) and all three (the FP16-storage operator).
+// Arithmetic is FP32 throughout.  Forms used: 0 (FP32), HM_IN16 (the
+// mixed-precision MBIR: FP16 gradient, FP32 spectrum and intermediates)
+// and all three (the FP16-storage operator).
 constexpr int HM_IN16 = 1, HM_WT16 = 2, HM_SPEC16 = 4;
 
 template <class P, int CM, int HM>
@@ -890,7 +890,7 @@ bool toeplitz_apply_ct(int M, const float* x, const float* b, float* out,
 }
 
 bool toeplitz_apply_ct_half(int M, const void* x16, float* out, void* wt,
-                            const void* spec16, int n, int H, int rp,
+                            const void* spec, int n, int H, int rp,
                             int nslices, double* partials, hipStream_t stream,
                             double img_b, double spec_b,
                             const Psf::StepCommit* sc,
@@ -898,11 +898,11 @@ bool toeplitz_apply_ct_half(int M, const void* x16, float* out, void* wt,
                             const ToeplitzStreams* ts) {
   if (wt16)
     return apply_ct<HM_IN16 | HM_WT16 | HM_SPEC16>(
-        M, x16, nullptr, out, wt, spec16, n, H, rp, nslices, 0, partials,
+        M, x16, nullptr, out, wt, spec, n, H, rp, nslices, 0, partials,
         stream, img_b, spec_b, sc, &half, ts);
-  return apply_ct<HM_IN16 | HM_SPEC16>(M, x16, nullptr, out, wt, spec16, n, H,
-                                       rp, nslices, 0, partials, stream, img_b,
-                                       spec_b, sc, &half, ts);
+  return apply_ct<HM_IN16>(M, x16, nullptr, out, wt, spec, n, H, rp,
+                           nslices, 0, partials, stream, img_b, spec_b, sc,
+                           &half, ts);
 }
 
 }  // namespace tc

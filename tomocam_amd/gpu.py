@@ -469,14 +469,13 @@ def pad2d(img: torch.Tensor, padding: int) -> torch.Tensor:
     return out
 
 
-# Accepted iterations between FP32 recomputations of T sol.  FP32: the
-# recurrence T sol += s T g drifts by FP32 rounding only; at 64 x 2048^2
-# and 100 iterations 25 instead of 10 is 3.5 % faster and moves the result
-# against FP64 from 6.9e-7 to 7.4e-7.  Mixed: T g also carries the FP16
-# spectrum's rounding, which the recurrence accumulates, so it keeps 10
-# (with 25 a 256^2, 20-iteration case drifts 1.0e-4 from FP32).
+# Accepted iterations between FP32 recomputations of T sol, in both forms:
+# the recurrence T sol += s T g drifts by FP32 rounding only (the mixed
+# form's T g reads the same FP16-rounded gradient as the step, with the
+# FP32 spectrum and intermediates).  At 64 x 2048^2 and 100 iterations 25
+# instead of 10 is 3.5 % faster and moves the result against FP64 from
+# 6.9e-7 to 7.4e-7 (FP32) and from 6.46e-6 to 6.48e-6 (mixed).
 REFRESH_EVERY = 25
-MIXED_REFRESH = 10
 
 
 def mbir(sino: torch.Tensor, angles, center: float, num_iters: int = 100,
@@ -490,10 +489,10 @@ def mbir(sino: torch.Tensor, angles, center: float, num_iters: int = 100,
     info) with info = lipschitz, step, sino_norm, max_sino, padded width and
     accepted iterations.
 
-    ``precision="mixed"`` runs the mixed-precision form (FP16 gradient and
-    Toeplitz intermediates, FP32 state and FFT arithmetic, FP64 dots, T sol
-    refreshed in FP32 every ``refresh_every`` accepted iterations: default
-    10 here, 25 in the FP32 form); info then also has ``saturated`` (slice-attempts whose FP16
+    ``precision="mixed"`` runs the mixed-precision form (FP16 gradient,
+    FP32 spectrum, intermediates, state and FFT arithmetic, FP64 dots, T sol
+    refreshed in FP32 every ``refresh_every`` accepted iterations, default
+    25 in both forms); info then also has ``saturated`` (slice-attempts whose FP16
     gradient clipped) and ``backtracks``.  ``lipschitz_iters > 0`` (mixed
     only) takes L from that many power-iteration steps on T instead of
     max(T 1).
@@ -513,8 +512,7 @@ def mbir(sino: torch.Tensor, angles, center: float, num_iters: int = 100,
     if lipschitz_iters and precision != "mixed":
         raise ValueError("lipschitz_iters is a mixed-precision option")
     if refresh_every is None:
-        refresh_every = MIXED_REFRESH if precision == "mixed" else \
-            REFRESH_EVERY
+        refresh_every = REFRESH_EVERY
     sino = _as3d(sino)
     S, P, N = sino.shape
     ang = _angles(angles)

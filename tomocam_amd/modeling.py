@@ -28,9 +28,9 @@ def MBIR(sinogram, angles, center, num_iters=100, smoothness=0.01, tol=1.0E-04,
     sinogram normalised by its maximum, like the reference).
 
     ``precision="mixed"`` (keyword only; the reference is FP32 throughout)
-    selects the mixed-precision form: FP16 gradient and Toeplitz
-    intermediates, FP32 FFTs and state, FP64 dots, T x refreshed in FP32
-    every 10 iterations.
+    selects the mixed-precision form: FP16 gradient, FP32 spectrum,
+    intermediates, FFTs and state, FP64 dots, T x refreshed in FP32 every
+    25 iterations.
     """
     if sinogram.dtype != np.float32:
         sinogram = sinogram.astype(np.float32)
