@@ -1,10 +1,11 @@
 """Mixed-precision form on the GPU (through the C-ABI).
 
-The form stores the gradient in FP16 under a per-slice power-of-two scale,
-stores the Toeplitz intermediates and the PSF spectrum in FP16, computes
-every FFT butterfly and the spreading in FP32, keeps the iterate, T x and b
-in FP32, sums the line-search dots in FP64 and recomputes T sol in FP32
-every 10 accepted iterations.
+The MBIR form stores the gradient in FP16 under a per-slice power-of-two
+scale, computes every FFT butterfly and the spreading in FP32 with FP32
+spectrum and Toeplitz intermediates, keeps the iterate, T x and b in FP32,
+sums the line-search dots in FP64 and recomputes T sol in FP32 every 25
+accepted iterations.  The FP16-storage Toeplitz operator (tc_toeplitz_apply
+_mixed) also stores its intermediates and the PSF spectrum in FP16.
 
 Tolerances:
 * the FP16-storage Toeplitz operator: relative L2 <= 1e-3 against the FP64

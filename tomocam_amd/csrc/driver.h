@@ -32,7 +32,7 @@ struct Mbir2Options {
   float p = 1.2f;
   int refresh_every = 25;
   int verbose = 0;
-  // mixed precision (FP16 gradient and spectrum; see Run2Options::mixed)
+  // mixed precision (FP16 gradient; see Run2Options::mixed)
   bool mixed = false;
   // > 0: L from that many power-iteration steps on T (FP64 dots), starting
   // from the all-ones image, instead of the reference's max(T 1)
