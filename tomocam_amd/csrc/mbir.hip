@@ -41,13 +41,16 @@ struct RhoParams {
   float qmp;  // q - p
 };
 
-// rho'(d) * w for a neighbour of weight w (ws2 = w / sigma^2)
+// rho'(d) * w for a neighbour of weight w (ws2 = w / sigma^2):
+//   w d / sigma^2 * r (2 - (q-p) t r),  t = (|d|/sigma)^(q-p), r = 1/(c + t)
+// with t r = 1 - c r:  r (2 - (q-p) + (q-p) c r), one FMA and one multiply
+// after the reciprocal (a sum of two non-negative terms: no cancellation)
 __device__ __forceinline__ float wdrho(float d, float ws2, const RhoParams& P) {
   const float a = fabsf(d) * P.inv_sigma;
   // (|d|/sigma)^(q-p) = 2^((q-p) log2(|d|/sigma)); log2(0) = -inf -> t = 0
   const float t = __builtin_amdgcn_exp2f(P.qmp * __builtin_amdgcn_logf(a));
   const float r = __builtin_amdgcn_rcpf(MRF_C + t);
-  return d * ws2 * r * (2.0f - P.qmp * t * r);
+  return (d * ws2) * (r * fmaf(P.qmp * MRF_C, r, 2.0f - P.qmp));
 }
 
 struct QArgs {
