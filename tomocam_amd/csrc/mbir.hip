@@ -38,7 +38,14 @@ __device__ __forceinline__ unsigned pack_h2(__half a, __half b) {
 struct RhoParams {
   float inv_sigma;
   float inv_sigma2;
-  float qmp;  // q - p
+  float qmp;   // q - p
+  float klog;  // (q - p) log2(1 / sigma)
+};
+
+// A neighbour class's constants for the stencil's rho' (ws2 = w / sigma^2):
+// a = ws2 (q-p) c, b = ws2 (2 - (q-p))
+struct RhoW {
+  float a, b;
 };
 
 // rho'(d) * w for a neighbour of weight w (ws2 = w / sigma^2):
@@ -51,6 +58,16 @@ __device__ __forceinline__ float wdrho(float d, float ws2, const RhoParams& P) {
   const float t = __builtin_amdgcn_exp2f(P.qmp * __builtin_amdgcn_logf(a));
   const float r = __builtin_amdgcn_rcpf(MRF_C + t);
   return (d * ws2) * (r * fmaf(P.qmp * MRF_C, r, 2.0f - P.qmp));
+}
+
+// The same with the neighbour weight folded into W and 1/sigma into the
+// exponent: t = 2^((q-p) log2|d| + (q-p) log2(1/sigma)); nine VALU
+// operations per pair term (the stencil's inner loop)
+__device__ __forceinline__ float wdrho_w(float d, RhoW W, const RhoParams& P) {
+  const float t = __builtin_amdgcn_exp2f(
+      fmaf(P.qmp, __builtin_amdgcn_logf(fabsf(d)), P.klog));
+  const float r = __builtin_amdgcn_rcpf(MRF_C + t);
+  return d * (r * fmaf(W.a, r, W.b));
 }
 
 struct QArgs {
@@ -71,7 +88,7 @@ struct QArgs {
   int zchunk;
   float beta;
   RhoParams rp;
-  float ws2_face, ws2_edge, ws2_corner;
+  RhoW w_face, w_edge, w_corner;
 };
 
 // Pair-symmetric stencil.  rho' is odd, so the term a voxel a receives
@@ -131,7 +148,7 @@ __device__ __forceinline__ float shift_x(float v) {
 template <int DY, int DX, bool FWD>
 __device__ __forceinline__ void qpair(const float (&lo)[QJ],
                                       const float (&hi)[QJ], float (&glo)[QJ],
-                                      float (&ghi)[QJ], float ws2,
+                                      float (&ghi)[QJ], RhoW W,
                                       const RhoParams& P) {
   constexpr int jlo = FWD ? (DY > 0 ? 0 : 1) : 1 - DY;
   constexpr int jhi = FWD ? (DY < 0 ? QROWS + 1 : QROWS) : QROWS - DY;
@@ -139,7 +156,7 @@ __device__ __forceinline__ void qpair(const float (&lo)[QJ],
 #pragma unroll
   for (int j = jlo; j <= jhi; ++j) {
     // V(a + o): column x + DX of plane hi
-    phi[j] = wdrho(lo[j] - shift_x<-DX>(hi[j + DY]), ws2, P);
+    phi[j] = wdrho_w(lo[j] - shift_x<-DX>(hi[j + DY]), W, P);
   }
   if constexpr (FWD) {
 #pragma unroll
@@ -164,24 +181,24 @@ __device__ __forceinline__ void qcross(const float (&lo)[QJ],
                                        const float (&hi)[QJ],
                                        float (&glo)[QJ], float (&ghi)[QJ],
                                        const QArgs& a) {
-  qpair<0, 0, FWD>(lo, hi, glo, ghi, a.ws2_face, a.rp);
-  qpair<1, 0, FWD>(lo, hi, glo, ghi, a.ws2_edge, a.rp);
-  qpair<-1, 0, FWD>(lo, hi, glo, ghi, a.ws2_edge, a.rp);
-  qpair<0, 1, FWD>(lo, hi, glo, ghi, a.ws2_edge, a.rp);
-  qpair<0, -1, FWD>(lo, hi, glo, ghi, a.ws2_edge, a.rp);
-  qpair<1, 1, FWD>(lo, hi, glo, ghi, a.ws2_corner, a.rp);
-  qpair<1, -1, FWD>(lo, hi, glo, ghi, a.ws2_corner, a.rp);
-  qpair<-1, 1, FWD>(lo, hi, glo, ghi, a.ws2_corner, a.rp);
-  qpair<-1, -1, FWD>(lo, hi, glo, ghi, a.ws2_corner, a.rp);
+  qpair<0, 0, FWD>(lo, hi, glo, ghi, a.w_face, a.rp);
+  qpair<1, 0, FWD>(lo, hi, glo, ghi, a.w_edge, a.rp);
+  qpair<-1, 0, FWD>(lo, hi, glo, ghi, a.w_edge, a.rp);
+  qpair<0, 1, FWD>(lo, hi, glo, ghi, a.w_edge, a.rp);
+  qpair<0, -1, FWD>(lo, hi, glo, ghi, a.w_edge, a.rp);
+  qpair<1, 1, FWD>(lo, hi, glo, ghi, a.w_corner, a.rp);
+  qpair<1, -1, FWD>(lo, hi, glo, ghi, a.w_corner, a.rp);
+  qpair<-1, 1, FWD>(lo, hi, glo, ghi, a.w_corner, a.rp);
+  qpair<-1, -1, FWD>(lo, hi, glo, ghi, a.w_corner, a.rp);
 }
 
 // the 4 in-plane forward pairs: (0,0,+1), (0,+1,-1), (0,+1,0), (0,+1,+1)
 __device__ __forceinline__ void qplane(const float (&v)[QJ], float (&g)[QJ],
                                        const QArgs& a) {
-  qpair<0, 1, true>(v, v, g, g, a.ws2_face, a.rp);
-  qpair<1, 0, true>(v, v, g, g, a.ws2_face, a.rp);
-  qpair<1, 1, true>(v, v, g, g, a.ws2_edge, a.rp);
-  qpair<1, -1, true>(v, v, g, g, a.ws2_edge, a.rp);
+  qpair<0, 1, true>(v, v, g, g, a.w_face, a.rp);
+  qpair<1, 0, true>(v, v, g, g, a.w_face, a.rp);
+  qpair<1, 1, true>(v, v, g, g, a.w_edge, a.rp);
+  qpair<1, -1, true>(v, v, g, g, a.w_edge, a.rp);
 }
 
 // y (= sol, or sol + beta (sol - xprev)) of plane z at the strip's
@@ -371,14 +388,20 @@ static RhoParams make_rho(float sigma, float p) {
   r.inv_sigma = 1.0f / sigma;
   r.inv_sigma2 = 1.0f / (sigma * sigma);
   r.qmp = 2.0f - p;
+  r.klog = (float)((2.0 - (double)p) * std::log2(1.0 / (double)sigma));
   return r;
 }
 
 static void set_rho(QArgs& a, float sigma, float p) {
   a.rp = make_rho(sigma, p);
-  a.ws2_face = W_FACE * a.rp.inv_sigma2;
-  a.ws2_edge = W_EDGE * a.rp.inv_sigma2;
-  a.ws2_corner = W_CORNER * a.rp.inv_sigma2;
+  const float qmp = a.rp.qmp;
+  auto cls = [&](float w) {
+    const float ws2 = w * a.rp.inv_sigma2;
+    return RhoW{ws2 * qmp * MRF_C, ws2 * (2.0f - qmp)};
+  };
+  a.w_face = cls(W_FACE);
+  a.w_edge = cls(W_EDGE);
+  a.w_corner = cls(W_CORNER);
 }
 
 // z chunking: a chunk costs its planes plus ~0.7 plane for the pairs with
