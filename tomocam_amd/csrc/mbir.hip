@@ -49,21 +49,12 @@ struct RhoW {
 };
 
 // rho'(d) * w for a neighbour of weight w (ws2 = w / sigma^2):
-//   w d / sigma^2 * r (2 - (q-p) t r),  t = (|d|/sigma)^(q-p), r = 1/(c + t)
-// with t r = 1 - c r:  r (2 - (q-p) + (q-p) c r), one FMA and one multiply
-// after the reciprocal (a sum of two non-negative terms: no cancellation)
-__device__ __forceinline__ float wdrho(float d, float ws2, const RhoParams& P) {
-  const float a = fabsf(d) * P.inv_sigma;
-  // (|d|/sigma)^(q-p) = 2^((q-p) log2(|d|/sigma)); log2(0) = -inf -> t = 0
-  const float t = __builtin_amdgcn_exp2f(P.qmp * __builtin_amdgcn_logf(a));
-  const float r = __builtin_amdgcn_rcpf(MRF_C + t);
-  return (d * ws2) * (r * fmaf(P.qmp * MRF_C, r, 2.0f - P.qmp));
-}
-
-// The same with the neighbour weight folded into W and 1/sigma into the
-// exponent: t = 2^((q-p) log2|d| + (q-p) log2(1/sigma)); nine VALU
-// operations per pair term (the stencil's inner loop)
-__device__ __forceinline__ float wdrho_w(float d, RhoW W, const RhoParams& P) {
+//   ws2 d r (2 - (q-p) t r),  t = (|d|/sigma)^(q-p), r = 1/(c + t),
+// with t r = 1 - c r:  d r (W.b + W.a r), W.a = ws2 (q-p) c,
+// W.b = ws2 (2 - (q-p)) (a sum of non-negative terms: no cancellation), and
+// t = 2^((q-p) log2|d| + (q-p) log2(1/sigma)): nine VALU operations per
+// pair term; log2(0) = -inf gives t = 0
+__device__ __forceinline__ float wdrho(float d, RhoW W, const RhoParams& P) {
   const float t = __builtin_amdgcn_exp2f(
       fmaf(P.qmp, __builtin_amdgcn_logf(fabsf(d)), P.klog));
   const float r = __builtin_amdgcn_rcpf(MRF_C + t);
@@ -156,7 +147,7 @@ __device__ __forceinline__ void qpair(const float (&lo)[QJ],
 #pragma unroll
   for (int j = jlo; j <= jhi; ++j) {
     // V(a + o): column x + DX of plane hi
-    phi[j] = wdrho_w(lo[j] - shift_x<-DX>(hi[j + DY]), W, P);
+    phi[j] = wdrho(lo[j] - shift_x<-DX>(hi[j + DY]), W, P);
   }
   if constexpr (FWD) {
 #pragma unroll
@@ -501,7 +492,9 @@ qggmrf_free_kernel(const float* __restrict__ x, float* g, double* partials,
           const float w = k == 1 ? W_FACE : (k == 2 ? W_EDGE : W_CORNER);
           const float d = xv - x[((long)zz * R + rr) * C + cc];
           if (MODE == 0) {
-            sum += wdrho(d, w * rp.inv_sigma2, rp);
+            const float ws2 = w * rp.inv_sigma2;
+            sum += wdrho(d, RhoW{ws2 * rp.qmp * MRF_C, ws2 * (2.0f - rp.qmp)},
+                           rp);
           } else {
             acc += (double)(w * rho_value(d, rp));
           }
