@@ -389,10 +389,13 @@ def extra_workloads(torch, gpu, phantom):
         torch.cuda.synchronize()
         return None, {"loss": float(loss.detach())}
 
-    el, res, _ = timed_steps(torch, gpu, step, 3, 1, [], "autograd-8x512")
+    # a step is ~1 ms: 100 timed steps keep the host's jitter out of it
+    nsteps = 100
+    el, res, _ = timed_steps(torch, gpu, step, nsteps, 5, [], "autograd-8x512")
     out["autograd-8x512"] = {"workload": w["label"],
-                             "slices_per_s": w["slices"] * 3 / el,
-                             "ms_per_step": el / 3 * 1e3}
+                             "slices_per_s": w["slices"] * nsteps / el,
+                             "ms_per_step": el / nsteps * 1e3,
+                             "steps": nsteps}
     torch.cuda.empty_cache()
     return out
 
