@@ -228,9 +228,18 @@ def cpu_baseline(sino_host, ang, w, total_slices, sample_slices,
     }
 
 
+# The HIP-event kernel timers bracket one launch in KTIMER_STRIDE of each
+# class during the timed steps: a timing event costs about 1 us of GPU time,
+# which at one-slice Toeplitz groups (60 us launches, two streams) slowed the
+# 64 x 2048^2 loop by 2.4 % when every launch was bracketed.  3 is coprime to
+# the power-of-two group counts, so every group position is sampled in turn.
+KTIMER_STRIDE = 3
+
+
 def timed_steps(torch, gpu, step, steps, warmup, classes, what):
     """`warmup` untimed calls, then `steps` timed ones with the kernel
-    classes' HIP-event timers on; returns (seconds, last result, kernels)."""
+    classes' HIP-event timers on (one launch in KTIMER_STRIDE of each class);
+    returns (seconds, last result, kernels)."""
     res = None
     def brief(r):
         return r[1] if isinstance(r[1], dict) else f"{len(r[1])} results"
@@ -238,6 +247,7 @@ def timed_steps(torch, gpu, step, steps, warmup, classes, what):
     for i in range(warmup):
         res = step()
         log(f"{what}: warmup {i} done: {brief(res)}")
+    gpu.ktimer_stride(KTIMER_STRIDE)
     for k in classes:
         gpu.ktimer_collect(k)  # discard earlier records
         gpu.ktimer_enable(k, True)
@@ -253,9 +263,11 @@ def timed_steps(torch, gpu, step, steps, warmup, classes, what):
         gpu.ktimer_enable(k, False)
         if cnt:
             kernels[gpu.KERNEL_CLASSES[k]] = dict(
-                launches=cnt, total_ms=ms, avg_ms=ms / cnt,
+                launches_timed=cnt, timer_stride=KTIMER_STRIDE,
+                total_ms=ms, avg_ms=ms / cnt,
                 bytes_per_launch=nbytes / cnt,
                 gbs=nbytes / (ms * 1e-3) / 1e9)
+    gpu.ktimer_stride(1)
     log(f"{what}: {steps} timed steps in {elapsed:.3f} s; info {brief(res)}")
     return elapsed, res, kernels
 
@@ -282,11 +294,13 @@ def roofline_of(kernels, workload):
 def step_roofline(kernels, elapsed, steps):
     """Whole-step rate: the algorithmic HBM bytes of every timed kernel class
     per step over the wall time of a step (set-up included).  Unlike the
-    per-kernel figure it credits kernels that overlap on two streams."""
+    per-kernel figure it credits kernels that overlap on two streams.  The
+    launches of a class are the timed ones times the timer stride (exact to
+    within stride - 1 launches of thousands)."""
     if not kernels:
         return None
-    nbytes = sum(k["bytes_per_launch"] * k["launches"] for k in
-                 kernels.values()) / steps
+    nbytes = sum(k["bytes_per_launch"] * k["launches_timed"] * k["timer_stride"]
+                 for k in kernels.values()) / steps
     ach = nbytes / (elapsed / steps) / 1e9
     return {"achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": ach / PEAK_HBM_GBS, "bytes_per_step": nbytes,

@@ -452,6 +452,13 @@ int tc_pad2d_f64(const double* d_img, double* d_out, int nslices, int n,
  * are bracketed by HIP events on their stream. */
 int tc_ktimer_enable(int kernel_class, int on);
 
+/* Time one launch in `stride` (>= 1, default 1) of each enabled class; the
+ * others run unbracketed.  Each timing event costs about 1 us of GPU time,
+ * which is several per cent of a loop of 60 us launches; a stride coprime
+ * to the launches per application (3 for power-of-two slice groups) samples
+ * every launch position in turn.  Resets the per-class launch counters. */
+int tc_ktimer_stride(int stride);
+
 /* Sum of the event-timed durations (ms), number of launches and algorithmic
  * HBM bytes (each input read once, each output written once) of a class
  * since the previous collect. */

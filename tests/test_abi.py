@@ -55,6 +55,9 @@ def test_error_path_without_gpu():
     with pytest.raises(ValueError, match="sigma"):
         import ctypes as C
         _lib.call("tc_add_tv_hessian", None, 0, -1.0, None)
+    with pytest.raises(ValueError, match="stride must be >= 1"):
+        _lib.call("tc_ktimer_stride", 0)
+    _lib.call("tc_ktimer_stride", 1)
 
 
 def test_tv_hessian_value_matches_oracle():

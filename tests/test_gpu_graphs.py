@@ -166,6 +166,30 @@ def test_graphs_off_with_kernel_timing(torch_cuda, problem):
         _timers_off(gpu)
 
 
+def test_kernel_timer_stride(torch_cuda, problem):
+    """A stride k times one launch in k of a class: the counts of two
+    identical runs are n and ceil(n / k), with the same bytes per launch."""
+    from tomocam_amd import gpu
+    psf, xt, b, ynorm, step = problem
+    announce("run2 with kernel timing, stride 1 then 3")
+    _timers_off(gpu)
+    res = []
+    try:
+        for stride in (1, 3):
+            gpu.ktimer_stride(stride)
+            gpu.ktimer_enable(1, True)
+            gpu.run2(psf, 0.5 * b, b, 10, step, 100.0, 1.2, ynorm)
+            ms, cnt, nb = gpu.ktimer_collect(1)
+            gpu.ktimer_enable(1, False)
+            res.append((ms, cnt, nb))
+    finally:
+        gpu.ktimer_stride(1)
+        _timers_off(gpu)
+    (ms1, n1, b1), (ms3, n3, b3) = res
+    assert n1 > 0 and n3 == -(-n1 // 3)
+    assert ms3 > 0 and b3 / n3 == pytest.approx(b1 / n1, rel=1e-12)
+
+
 def test_concurrent_batch_with_graphs_requested(torch_cuda):
     """Concurrent batch workers keep graphs off even when requested (a
     capture in one worker would break the others' synchronous copies) and

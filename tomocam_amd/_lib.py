@@ -110,6 +110,7 @@ SIGNATURES = {
     "tc_alloc_stats": [c_long_p, c_long_p, c_double_p],
     "tc_release_cached": [],
     "tc_ktimer_enable": [C.c_int, C.c_int],
+    "tc_ktimer_stride": [C.c_int],
     "tc_ktimer_collect": [C.c_int, c_double_p, c_long_p, c_double_p],
 }
 

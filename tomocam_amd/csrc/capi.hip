@@ -16,6 +16,7 @@
 
 namespace tc {
 void ktimer_enable(int kc, bool on);
+void ktimer_set_stride(int stride);
 void ktimer_collect(int kc, double* total_ms, long* count, double* bytes);
 }  // namespace tc
 
@@ -675,6 +676,10 @@ int tc_mbir2_host(const float* h_sino, float* h_recon, int nslices, int nproj,
 
 int tc_ktimer_enable(int kernel_class, int on) {
   return guarded([&] { tc::ktimer_enable(kernel_class, on != 0); });
+}
+
+int tc_ktimer_stride(int stride) {
+  return guarded([&] { tc::ktimer_set_stride(stride); });
 }
 
 int tc_graph_replays(long* count) {

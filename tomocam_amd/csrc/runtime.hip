@@ -213,20 +213,48 @@ void arena_trim() {
 // ---------------------------------------------------------------------------
 
 // Event pairs recorded around launches of enabled kernel classes; resolved
-// (and released) by ktimer_collect.
+// by ktimer_collect, which returns the events to a per-device pool (creating
+// two events per timed launch measurably slowed loops of short launches).
+struct KPair {
+  hipEvent_t a, b;
+  int dev;
+};
+
 struct KTimer {
   std::mutex mu;
   bool enabled[KC_COUNT] = {};
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs[KC_COUNT];
+  std::vector<KPair> pairs[KC_COUNT];
   double bytes[KC_COUNT] = {};
+  std::map<int, std::vector<hipEvent_t>> pool;  // resolved events, by device
+  // one launch in `stride` of each class is timed (launch counters per class)
+  std::atomic<int> stride{1};
+  std::atomic<long> launches[KC_COUNT] = {};
 };
 
-// the event opened by ktimer_begin, per thread (concurrent callers)
+// the event opened by ktimer_begin (and its device), per thread (concurrent
+// callers)
 static thread_local hipEvent_t t_open[KC_COUNT] = {};
+static thread_local int t_open_dev[KC_COUNT] = {};
 
 static KTimer& kt() {
   static KTimer k;
   return k;
+}
+
+// an event of device dev from the pool, else a new one
+static hipEvent_t ktimer_event(KTimer& k, int dev) {
+  {
+    std::lock_guard<std::mutex> lk(k.mu);
+    auto& v = k.pool[dev];
+    if (!v.empty()) {
+      hipEvent_t e = v.back();
+      v.pop_back();
+      return e;
+    }
+  }
+  hipEvent_t e;
+  TC_HIP(hipEventCreate(&e));
+  return e;
 }
 
 void ktimer_enable(int kc, bool on) {
@@ -243,23 +271,34 @@ bool ktimer_active() {
   return false;
 }
 
+void ktimer_set_stride(int stride) {
+  TC_REQUIRE(stride >= 1, "kernel timer: stride must be >= 1");
+  KTimer& k = kt();
+  k.stride.store(stride);
+  for (auto& c : k.launches) c.store(0);
+}
+
 void ktimer_begin(int kc, hipStream_t stream) {
   KTimer& k = kt();
   if (!k.enabled[kc]) return;
-  hipEvent_t e;
-  TC_HIP(hipEventCreate(&e));
+  const int st = k.stride.load();
+  if (st > 1 && k.launches[kc].fetch_add(1) % st != 0) return;
+  int dev = 0;
+  TC_HIP(hipGetDevice(&dev));
+  hipEvent_t e = ktimer_event(k, dev);
   TC_HIP(hipEventRecord(e, stream));
   t_open[kc] = e;
+  t_open_dev[kc] = dev;
 }
 
 void ktimer_end(int kc, hipStream_t stream, double bytes) {
   KTimer& k = kt();
   if (!k.enabled[kc] || !t_open[kc]) return;
-  hipEvent_t e;
-  TC_HIP(hipEventCreate(&e));
+  const int dev = t_open_dev[kc];
+  hipEvent_t e = ktimer_event(k, dev);
   TC_HIP(hipEventRecord(e, stream));
   std::lock_guard<std::mutex> lk(k.mu);
-  k.pairs[kc].emplace_back(t_open[kc], e);
+  k.pairs[kc].push_back(KPair{t_open[kc], e, dev});
   k.bytes[kc] += bytes;
   t_open[kc] = nullptr;
 }
@@ -273,13 +312,14 @@ void ktimer_collect(int kc, double* total_ms, long* count, double* bytes) {
   double tot = 0.0;
   long cnt = 0;
   for (auto& pr : k.pairs[kc]) {
-    TC_HIP(hipEventSynchronize(pr.second));
+    TC_HIP(hipEventSynchronize(pr.b));
     float ms = 0.f;
-    TC_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+    TC_HIP(hipEventElapsedTime(&ms, pr.a, pr.b));
     tot += ms;
     ++cnt;
-    (void)hipEventDestroy(pr.first);
-    (void)hipEventDestroy(pr.second);
+    auto& v = k.pool[pr.dev];
+    v.push_back(pr.a);
+    v.push_back(pr.b);
   }
   k.pairs[kc].clear();
   *total_ms = tot;

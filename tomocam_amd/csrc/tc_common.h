@@ -213,6 +213,7 @@ enum KernelClass : int {
 // that class is enabled; cheap no-op otherwise.
 // `bytes` is the algorithmic HBM traffic of the launch (inputs read once,
 // outputs written once), accumulated for the roofline report.
+void ktimer_set_stride(int stride);
 void ktimer_begin(int kc, hipStream_t stream);
 void ktimer_end(int kc, hipStream_t stream, double bytes = 0.0);
 // True if timing is enabled for any kernel class.

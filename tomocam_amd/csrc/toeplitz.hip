@@ -4,6 +4,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <initializer_list>
 #include <map>
 #include <mutex>
 #include <utility>
@@ -42,6 +45,13 @@ static std::vector<int> toeplitz_ct_candidates(int n) {
   std::vector<int> out;
   for (auto& x : c) out.push_back(x.second);
   return out;
+}
+
+// TOMOCAM_HIP_TUNE_LOG=1: the autotuners print each candidate's time to
+// stderr
+static bool tune_log() {
+  const char* e = std::getenv("TOMOCAM_HIP_TUNE_LOG");
+  return e && e[0] == '1';
 }
 
 // FFT lengths picked by Psf::tune_length, keyed by (device, n)
@@ -177,6 +187,10 @@ int Psf::tune_length(int nslices, hipStream_t stream) {
         TC_HIP(hipEventElapsedTime(&t, e0, e1));
         if (rep == 1 || (rep > 1 && t < ms)) ms = t;
       }
+      if (tune_log())
+        std::fprintf(stderr,
+                     "[tomocam] toeplitz tune n=%d, %d slices: length %d: "
+                     "%.3f ms\n", n_, S, L, (double)ms);
       if (best == 0 || ms < (1.0f - kMargin) * best_ms) {
         best = L;
         best_ms = ms;
@@ -518,7 +532,7 @@ int Psf::tune_groups(int nslices, bool half, hipStream_t stream) {
                        0, stream, gsc.as<float>(), (long)nslices, 1.0f);
     TC_HIP(hipGetLastError());
   }
-  // candidates: groups around the 144 MB rule's group r (r/2, r, 2r, capped
+  // candidates: groups around the 144 MB rule's group r (r, r/2, 2r, capped
   // at nslices) on one stream, and r/2 and r on two streams (groups
   // alternate between the caller's stream and a second one, so one group's
   // latency-bound row passes overlap the next group's VALU-bound column
@@ -526,12 +540,19 @@ int Psf::tune_groups(int nslices, bool half, hipStream_t stream) {
   // caller's stream, which waits for the second one.
   // Candidates in order of preference; a later one replaces the best so
   // far only when at least kMargin faster, so near-ties resolve the same
-  // way on every run and the configuration is reproducible.
+  // way on every run and the configuration is reproducible.  On two streams
+  // the smaller group comes first: the loop's applications (with the step
+  // commit and dot partials in the last pass) overlap better in small
+  // groups than this scratch application shows.  At 64 x 2908^2 one slice
+  // on two streams timed 0.5 % faster here than two slices, and the
+  // 100-iteration loop ran 3 % faster with it (34.4 against 33.5 slices/s).
   constexpr float kMargin = 0.02f;
   const int r = toeplitz_rule_group(nslices, H_, rp_, eb);
   std::vector<std::pair<int, int>> cands;
   for (int st : {1, 2})
-    for (int g : {r, std::max(1, r / 2), std::min(2 * r, nslices)}) {
+    for (int g : st == 1 ? std::initializer_list<int>{r, std::max(1, r / 2),
+                                                      std::min(2 * r, nslices)}
+                         : std::initializer_list<int>{std::max(1, r / 2), r}) {
       if (st == 2 && (g > r || 2 * g > nslices)) continue;
       bool seen = false;
       for (auto& c : cands) seen |= c.first == g && c.second == st;
@@ -561,6 +582,11 @@ int Psf::tune_groups(int nslices, bool half, hipStream_t stream) {
         TC_HIP(hipEventElapsedTime(&t, e0, e1));
         if (rep == 1 || (rep > 1 && t < ms)) ms = t;
       }
+      if (tune_log())
+        std::fprintf(stderr,
+                     "[tomocam] toeplitz tune n=%d, %d slices: group %d, "
+                     "%d stream(s): %.3f ms\n", n_, nslices, g, st,
+                     (double)ms);
       if (best == 0 || ms < (1.0f - kMargin) * best_ms) {
         best = g;
         best_st = st;

@@ -287,6 +287,14 @@ def ktimer_enable(kernel_class: int, on: bool = True) -> None:
     _lib.call("tc_ktimer_enable", int(kernel_class), int(bool(on)))
 
 
+def ktimer_stride(stride: int = 1) -> None:
+    """Time one launch in `stride` of each enabled class (timing events cost
+    GPU time per launch; 1 times every launch)."""
+    if int(stride) < 1:
+        raise ValueError("stride must be >= 1")
+    _lib.call("tc_ktimer_stride", int(stride))
+
+
 KERNEL_CLASSES = {
     0: "toeplitz_rows_fwd",
     1: "toeplitz_cols",
