@@ -118,6 +118,38 @@ def test_mbir2_mixed_vs_oracle(torch_cuda):
     assert e_mixed < 1e-4
 
 
+def test_mixed_slice_groups_and_streams(torch_cuda, monkeypatch):
+    """The mixed-precision MBIR (FP16 gradient, FP32 intermediates) and the
+    FP16-intermediate operator (per-group pass-B scales) on 5 slices in
+    groups of 1 or 2, on one stream and round-robin over two or three,
+    against one group on one stream."""
+    torch = torch_cuda
+    from tomocam_amd import gpu
+    N, P, S, iters, sigma = 256, 120, 5, 20, 100.0
+    ang, sino = _sino(N, P, S, 23)
+    st_t = torch.tensor(sino, dtype=torch.float32).cuda()
+    ps = O.psf(PH.angles(180), 362)
+    psf = gpu.PointSpreadFunction(
+        psf_array=torch.tensor(ps, dtype=torch.float32).cuda())
+    x = torch.tensor(np.random.default_rng(5).standard_normal((S, 362, 362)),
+                     dtype=torch.float32).cuda()
+    monkeypatch.setenv("TOMOCAM_HIP_AUTOTUNE", "0")
+    recs, convs = {}, {}
+    for groups, streams in (("0", "1"), ("2", "2"), ("1", "3"), ("2", "3")):
+        monkeypatch.setenv("TOMOCAM_HIP_TOEPLITZ_SLICES", groups)
+        monkeypatch.setenv("TOMOCAM_HIP_TOEPLITZ_STREAMS", streams)
+        announce(f"mixed slice groups {groups} streams {streams}")
+        rec, info = gpu.mbir(st_t, ang, N / 2, iters, sigma,
+                             precision="mixed")
+        assert info["saturated"] == 0
+        recs[(groups, streams)] = rec.cpu().numpy()
+        convs[(groups, streams)] = psf.convolve_mixed(x).cpu().numpy()
+    one, cone = recs[("0", "1")], convs[("0", "1")]
+    for key in (("2", "2"), ("1", "3"), ("2", "3")):
+        assert rel_l2(recs[key], one) < 1e-6, key
+        assert rel_l2(convs[key], cone) < 1e-6, key
+
+
 def test_mbir2_mixed_power_lipschitz(torch_cuda):
     """L from 20 power-iteration steps on T (FP64 dots) against the FP64
     oracle's 20 steps; the step is min(1, 1/L)."""

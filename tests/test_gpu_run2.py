@@ -171,8 +171,9 @@ def test_run2_fused_commit_with_retries(torch_cuda, n, S, mult, retries):
 
 def test_run2_slice_groups_and_two_streams(torch_cuda, monkeypatch):
     """The compile-time passes in several slice groups (5 slices in groups of
-    2: 2 + 2 + 1), on one stream and alternating over two, against one
-    group and against the FP64 oracle (padded width 362, FFT length 768)."""
+    2: 2 + 2 + 1, or of 1), on one stream and round-robin over two or three,
+    against one group and against the FP64 oracle (padded width 362, FFT
+    length 768)."""
     torch = torch_cuda
     ang = PH.angles(60).astype(np.float64)
     img = np.stack([PH.random_ellipses(256, 500 + s) for s in range(5)])
@@ -184,7 +185,8 @@ def test_run2_slice_groups_and_two_streams(torch_cuda, monkeypatch):
     ref = O.run2_linear(st.x0, T, st.sinoT, reg, st.sino_norm, iters, st.step)
     monkeypatch.setenv("TOMOCAM_HIP_AUTOTUNE", "0")
     outs = {}
-    for groups, streams in (("0", "1"), ("2", "1"), ("2", "2")):
+    for groups, streams in (("0", "1"), ("2", "1"), ("2", "2"), ("1", "3"),
+                            ("2", "3")):
         monkeypatch.setenv("TOMOCAM_HIP_TOEPLITZ_SLICES", groups)
         monkeypatch.setenv("TOMOCAM_HIP_TOEPLITZ_STREAMS", streams)
         announce(f"run2 slice groups {groups} streams {streams}")
@@ -194,3 +196,5 @@ def test_run2_slice_groups_and_two_streams(torch_cuda, monkeypatch):
     one = outs[("0", "1")]
     assert rel_l2(outs[("2", "1")], one) < 1e-6
     assert rel_l2(outs[("2", "2")], one) < 1e-6
+    assert rel_l2(outs[("1", "3")], one) < 1e-6
+    assert rel_l2(outs[("2", "3")], one) < 1e-6

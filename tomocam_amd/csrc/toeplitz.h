@@ -16,13 +16,15 @@
 
 namespace tc {
 
-// Second stream for the Toeplitz passes: slice groups alternate between the
-// caller's stream and `aux` (each with its own workspace half), so one
-// group's row passes (memory-latency bound) overlap the next group's column
-// pass (VALU bound).
+// Extra streams for the Toeplitz passes: slice groups go round-robin over
+// the caller's stream and n - 1 `aux` streams (each with its own workspace
+// slab), so one group's row passes (memory-latency bound) overlap another
+// group's column pass (VALU bound).
+constexpr int kMaxToeplitzStreams = 3;
 struct ToeplitzStreams {
-  hipStream_t aux;
-  hipEvent_t start, done;
+  int n;  // streams in use, 2 .. kMaxToeplitzStreams
+  hipStream_t aux[kMaxToeplitzStreams - 1];
+  hipEvent_t start, done[kMaxToeplitzStreams - 1];
 };
 
 class Psf {
@@ -125,7 +127,7 @@ class Psf {
   // the two-stream set-up when TOMOCAM_HIP_TOEPLITZ_STREAMS (or the tuner)
   // asks for two streams for this slice count, else nullptr
   const ToeplitzStreams* streams_for(int nslices, int elem_bytes);
-  ToeplitzStreams ts_{nullptr, nullptr, nullptr};
+  ToeplitzStreams ts_{};
   bool run_half(const void* g16, const float* gsc, float* out, int nslices,
                 double* dot_partials, const StepCommit* c, hipStream_t stream,
                 bool wt16);

@@ -395,9 +395,11 @@ void toeplitz_set_streams(int H, int rp, int elem_bytes, int nslices,
                           int streams);
 
 Psf::~Psf() {
-  if (ts_.aux) (void)hipStreamDestroy(ts_.aux);
+  for (int k = 0; k < kMaxToeplitzStreams - 1; ++k) {
+    if (ts_.aux[k]) (void)hipStreamDestroy(ts_.aux[k]);
+    if (ts_.done[k]) (void)hipEventDestroy(ts_.done[k]);
+  }
   if (ts_.start) (void)hipEventDestroy(ts_.start);
-  if (ts_.done) (void)hipEventDestroy(ts_.done);
 }
 
 bool Psf::single_stream(int nslices, bool half) const {
@@ -405,13 +407,17 @@ bool Psf::single_stream(int nslices, bool half) const {
 }
 
 const ToeplitzStreams* Psf::streams_for(int nslices, int elem_bytes) {
-  if (toeplitz_streams(H_, rp_, elem_bytes, nslices) < 2) return nullptr;
-  if (!ts_.aux) {
-    // default priority, non-blocking: ordered only by the events below
-    TC_HIP(hipStreamCreateWithFlags(&ts_.aux, hipStreamNonBlocking));
+  const int st = toeplitz_streams(H_, rp_, elem_bytes, nslices);
+  if (st < 2) return nullptr;
+  if (!ts_.start)
     TC_HIP(hipEventCreateWithFlags(&ts_.start, hipEventDisableTiming));
-    TC_HIP(hipEventCreateWithFlags(&ts_.done, hipEventDisableTiming));
-  }
+  for (int k = 0; k < st - 1; ++k)
+    if (!ts_.aux[k]) {
+      // default priority, non-blocking: ordered only by the events
+      TC_HIP(hipStreamCreateWithFlags(&ts_.aux[k], hipStreamNonBlocking));
+      TC_HIP(hipEventCreateWithFlags(&ts_.done[k], hipEventDisableTiming));
+    }
+  ts_.n = st;
   return &ts_;
 }
 
@@ -479,8 +485,8 @@ bool Psf::run_half(const void* g16, const float* gsc, float* out,
     TC_HIP(hipGetLastError());
   }
   reserve(nslices);
-  // pass B's output scales for two groups in flight (two-stream passes)
-  const int group = 2 * toeplitz_half_group(nslices, H_, rp_);
+  // pass B's output scales for one group in flight per stream
+  const int group = kMaxToeplitzStreams * toeplitz_half_group(nslices, H_, rp_);
   if (wt16 && (size_t)group > cinv_slices_) {
     cinv_.alloc(sizeof(float) * (size_t)group * toeplitz_ct_scales(M_, H_));
     cinv_slices_ = group;

@@ -663,10 +663,11 @@ void toeplitz_set_streams(int H, int rp, int elem_bytes, int nslices,
     g_streams_tuned.erase(key);
 }
 
-// TOMOCAM_HIP_TOEPLITZ_STREAMS, else the tuner's choice, else 1
+// TOMOCAM_HIP_TOEPLITZ_STREAMS (1 .. kMaxToeplitzStreams), else the
+// tuner's choice, else 1
 int toeplitz_streams(int H, int rp, int elem_bytes, int nslices) {
   const char* e = std::getenv("TOMOCAM_HIP_TOEPLITZ_STREAMS");
-  if (e) return std::atoi(e) >= 2 ? 2 : 1;
+  if (e) return std::max(1, std::min(std::atoi(e), kMaxToeplitzStreams));
   std::lock_guard<std::mutex> lk(g_group_mu);
   auto it = g_streams_tuned.find(group_key(H, rp, elem_bytes, nslices));
   return it != g_streams_tuned.end() ? it->second : 1;
@@ -727,19 +728,25 @@ static void run_ct(const void* x, const float* b, float* out, void* wt,
   const double ibx = (HM & HM_IN16) ? 0.5 * ib : ib;
   const double sbw = W16 ? 0.5 * sb : sb;
   const size_t xel = (HM & HM_IN16) ? 2 : 4;
-  const bool dual = ts != nullptr && nslices > group;
-  if (dual) {
+  // streams in use: group gi runs on stream gi % nst (0: the caller's) with
+  // workspace slab gi % nst.  Slab k is first used by group k, so a full
+  // group on it implies (k + 1) * group <= nslices: the slabs fit in the
+  // nslices-slice workspace.
+  const int nst = ts == nullptr
+                      ? 1
+                      : std::max(1, std::min(ts->n, (int)cdiv(nslices, group)));
+  if (nst > 1) {
     TC_HIP(hipEventRecord(ts->start, stream0));
-    TC_HIP(hipStreamWaitEvent(ts->aux, ts->start, 0));
+    for (int k = 0; k < nst - 1; ++k)
+      TC_HIP(hipStreamWaitEvent(ts->aux[k], ts->start, 0));
   }
   const long wslab = (long)group * H * rp;  // workspace elements per group
   for (int s0 = 0, gi = 0; s0 < nslices; s0 += group, ++gi) {
     const int ns = std::min(group, nslices - s0);
     const long o = (long)s0 * img;
-    // odd groups: the second stream and the second workspace half
-    const bool odd = dual && (gi & 1);
-    hipStream_t stream = odd ? ts->aux : stream0;
-    void* wtg = static_cast<char*>(wt) + (odd ? wslab * (W16 ? 4 : 8) : 0);
+    const int k = gi % nst;
+    hipStream_t stream = k ? ts->aux[k - 1] : stream0;
+    void* wtg = static_cast<char*>(wt) + (long)k * wslab * (W16 ? 4 : 8);
     CommitPtrs cp{};
     if (sc)
       cp = CommitPtrs{sc->sol + o, sc->xp ? sc->xp + o : nullptr,
@@ -750,7 +757,7 @@ static void run_ct(const void* x, const float* b, float* out, void* wt,
     if (half) {
       ha = *half;
       ha.gsc = half->gsc + s0;
-      if (odd) ha.cinv = half->cinv + (long)group * P::scales_per_slice(H);
+      ha.cinv = half->cinv + (long)k * group * P::scales_per_slice(H);
     }
     const void* xg = static_cast<const char*>(x) + xel * o;
     const float* bg = b ? b + o : nullptr;
@@ -800,9 +807,9 @@ static void run_ct(const void* x, const float* b, float* out, void* wt,
       ktimer_end(kc_inv, stream, ns * (sbw + ib * (1.0 + extra)));
     }
   }
-  if (dual) {
-    TC_HIP(hipEventRecord(ts->done, ts->aux));
-    TC_HIP(hipStreamWaitEvent(stream0, ts->done, 0));
+  for (int k = 0; k + 1 < nst; ++k) {
+    TC_HIP(hipEventRecord(ts->done[k], ts->aux[k]));
+    TC_HIP(hipStreamWaitEvent(stream0, ts->done[k], 0));
   }
 }
 

@@ -129,8 +129,8 @@ class PointSpreadFunction:
 
     def convolve_mixed(self, x: torch.Tensor, with_dot: bool = False):
         """T x with x rounded to FP16 under a per-slice power-of-two scale and
-        FP16 intermediates (the operator of the mixed-precision MBIR form);
-        optionally also x16 . T x16."""
+        FP16 intermediates (the mixed-precision MBIR form reads an FP16 input
+        but keeps FP32 intermediates); optionally also x16 . T x16."""
         x = self._check(x, "x")
         out = torch.empty_like(x)
         dot = C.c_double()
