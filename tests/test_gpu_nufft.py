@@ -132,3 +132,47 @@ def test_pre_post_processing_exact(torch_cuda, N, center):
     small = rng.standard_normal((2, N, N)).astype(np.float32)
     assert np.array_equal(gpu.pad2d(_t(torch, small), n2 - N).cpu().numpy(),
                           O.pad2d(small, n2 - N))
+
+
+def test_spread_chunk_autotuner(torch_cuda, capfd, monkeypatch):
+    """The spreading chunk (points per work item) is tuned on a geometry's
+    first type-1 call (each candidate logged with TOMOCAM_HIP_TUNE_LOG=1)
+    and kept for later calls; forced chunks (TOMOCAM_HIP_SPREAD_CHUNK) and
+    TOMOCAM_HIP_AUTOTUNE=0 skip the tuning.  Every chunk gives the exact
+    backprojection within 1e-5.  The angle counts are ones no other test
+    uses, so each geometry is new to the process."""
+    torch = torch_cuda
+    from tomocam_amd import gpu
+    N, S = 64, 2
+    rng = np.random.default_rng(77)
+    monkeypatch.setenv("TOMOCAM_HIP_TUNE_LOG", "1")
+    monkeypatch.delenv("TOMOCAM_HIP_AUTOTUNE", raising=False)
+    monkeypatch.delenv("TOMOCAM_HIP_SPREAD_CHUNK", raising=False)
+
+    def run(P):
+        ang = PH.angles(P)
+        sino = rng.standard_normal((S, P, N))
+        capfd.readouterr()
+        got = gpu.backproject(_t(torch, sino), ang).cpu().numpy()
+        lines = [l for l in capfd.readouterr().err.splitlines()
+                 if "spread tune" in l]
+        ref = O.backproject(sino, ang.astype(np.float64), exact=True)
+        assert rel_l2(got, ref) < 1e-5
+        return sino, ang, got, lines
+
+    announce("spread chunk autotuner: tuned geometry")
+    sino, ang, first, lines = run(211)
+    # the rule's chunk and the other two of 512, 1024, 2048
+    assert len(lines) == 3, lines
+    capfd.readouterr()
+    again = gpu.backproject(_t(torch, sino), ang).cpu().numpy()
+    assert "spread tune" not in capfd.readouterr().err
+    assert np.array_equal(first, again)
+    for P, chunk in ((212, "64"), (213, "160"), (214, "4096")):
+        announce(f"spread chunk forced to {chunk}")
+        monkeypatch.setenv("TOMOCAM_HIP_SPREAD_CHUNK", chunk)
+        assert run(P)[3] == []
+    monkeypatch.delenv("TOMOCAM_HIP_SPREAD_CHUNK")
+    announce("spread chunk with autotuning off")
+    monkeypatch.setenv("TOMOCAM_HIP_AUTOTUNE", "0")
+    assert run(215)[3] == []

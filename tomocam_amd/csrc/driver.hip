@@ -100,6 +100,8 @@ Mbir2Result mbir2(const float* sino, const float* x0, float* recon, int S,
   // the call's buffers come from (and go back to) the thread's arena; the
   // call ends with a stream synchronisation
   ArenaScope arena;
+  // no autotuning (here the NUFFT spreading chunk) unless the caller allows
+  AutotuneScope tune_scope(o.autotune);
   Mbir2Result res;
   const long nsino = (long)S * P * N;
 

@@ -50,13 +50,15 @@ class PolarNufft {
   int fine_batch_ = 0;
   float2* fine(int batch);
   // Spreading work list, built once per geometry: one item per (fine-grid
-  // tile, chunk of at most SPREAD_CHUNK points) so that the tiles near the
+  // tile, chunk of at most chunk_ points) so that the tiles near the
   // centre, which every ray crosses, are split over many workgroups.
   bool work_ready_ = false;
-  int nwork_ = 0, nslots_ = 0, nfix_ = 0;
+  int nwork_ = 0, nslots_ = 0, nfix_ = 0, chunk_ = 0;
+  std::vector<int> counts_;  // polar points per fine-grid tile
   DevBuf work_;  // int4: tile, first point, end point, scratch slot (-1: none)
   DevBuf fix_;   // int4: tile, first slot, slot count, 0
-  void build_work(hipStream_t stream);
+  void count_tiles(hipStream_t stream);
+  void make_work(int chunk);
 };
 
 // Centred DFT along rays by Bluestein's algorithm:

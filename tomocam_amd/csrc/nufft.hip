@@ -2,6 +2,7 @@
 // Bluestein ray DFT and the projectors built on them.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <map>
 #include <mutex>
@@ -711,9 +712,8 @@ float2* PolarNufft::fine(int batch) {
   return fine_.as<float2>();
 }
 
-void PolarNufft::build_work(hipStream_t stream) {
-  if (work_ready_) return;
-  ArenaPause no_arena;  // the plan's work lists, filled synchronously
+// Points per tile of this geometry (a device count, read back once).
+void PolarNufft::count_tiles(hipStream_t stream) {
   const int ntx = (int)cdiv(nf_, SP_T);
   const int ntiles = ntx * ntx;
   const int cidx = N_ / 2;
@@ -730,15 +730,22 @@ void PolarNufft::build_work(hipStream_t stream) {
   hipLaunchKernelGGL(spread_count_kernel, dim3(ntiles), dim3(256), 0, stream,
                      a, dcounts.as<int>());
   TC_HIP(hipGetLastError());
-  std::vector<int> counts(ntiles);
-  TC_HIP(hipMemcpyAsync(counts.data(), dcounts.as<void>(), sizeof(int) * ntiles,
-                        hipMemcpyDeviceToHost, stream));
+  counts_.assign(ntiles, 0);
+  TC_HIP(hipMemcpyAsync(counts_.data(), dcounts.as<void>(),
+                        sizeof(int) * ntiles, hipMemcpyDeviceToHost, stream));
   TC_HIP(hipStreamSynchronize(stream));
+}
+
+// The work and fix-up lists for `chunk` points per item, uploaded to the
+// plan's buffers (synchronous copies: the caller has no kernel reading the
+// previous lists in flight).
+void PolarNufft::make_work(int chunk) {
+  ArenaPause no_arena;  // the plan's work lists, filled synchronously
+  const int ntiles = (int)counts_.size();
   std::vector<int4> work, fix;
   int slot = 0;
-  const int chunk = spread_chunk(ntiles);
   for (int t = 0; t < ntiles; ++t) {
-    const int c = counts[t];
+    const int c = counts_[t];
     const int nch = std::max(1, (c + chunk - 1) / chunk);
     work.push_back(make_int4(t, 0, std::min(c, chunk), -1));
     if (nch > 1) {
@@ -770,6 +777,7 @@ void PolarNufft::build_work(hipStream_t stream) {
   nwork_ = (int)work.size();
   nslots_ = slot;
   nfix_ = (int)fix.size();
+  chunk_ = chunk;
   work_.alloc(sizeof(int4) * nwork_);
   TC_HIP(hipMemcpy(work_.as<void>(), work.data(), sizeof(int4) * nwork_,
                    hipMemcpyHostToDevice));
@@ -778,7 +786,19 @@ void PolarNufft::build_work(hipStream_t stream) {
     TC_HIP(hipMemcpy(fix_.as<void>(), fix.data(), sizeof(int4) * nfix_,
                      hipMemcpyHostToDevice));
   }
-  work_ready_ = true;
+}
+
+// Spreading chunks picked by the autotuner, keyed by (device, angles, ray
+// samples, fine grid, modes): later plans of the geometry, including those
+// of concurrent batch workers (which do not tune), use the choice.
+static std::mutex g_chunk_mu;
+static std::map<std::tuple<int, int, int, int, int>, int> g_chunk_tuned;
+
+static std::tuple<int, int, int, int, int> chunk_key(int P, int N, int nf,
+                                                     int nm) {
+  int dev = 0;
+  TC_HIP(hipGetDevice(&dev));
+  return std::make_tuple(dev, P, N, nf, nm);
 }
 
 // Slices per fine-grid batch: about 1 GB of fine grid, in whole groups of
@@ -801,18 +821,14 @@ static int fine_batch(int nf, int S) {
 void PolarNufft::type1_real(const float2* c, float* out, int S, float scale,
                             hipStream_t stream) {
   TC_REQUIRE(S >= 1, "NUFFT: need at least one slice");
-  build_work(stream);
   const int B = fine_batch(nf_, S);
   float2* fg = fine(B);
   const int ntx = (int)cdiv(nf_, SP_T);
   const unsigned ngroups_max = cdiv(B, SP_SB);
-  DevBuf scratch(sizeof(float2) * (size_t)std::max(nslots_, 1) * ngroups_max *
-                 SP_SB * SP_T * SP_T);
   const int cidx = N_ / 2;
   const double umax =
       std::max(cidx, N_ - 1 - cidx) * ((double)nf_ / (N_ - 1)) + 1.0;
-  for (int s0 = 0; s0 < S; s0 += B) {
-    const int nb = std::min(B, S - s0);
+  auto args = [&](int s0, int nb) {
     SpreadArgs a;
     a.c = c + (long)s0 * P_ * N_;
     a.fine = fg;
@@ -826,18 +842,96 @@ void PolarNufft::type1_real(const float2* c, float* out, int S, float scale,
     a.ntx = ntx;
     a.umax = umax;
     a.beta = beta_;
+    return a;
+  };
+  const size_t seg_lds = sizeof(int) * (2 * (size_t)P_ + 1);
+  // the fine grid of nb slices from the current work lists
+  auto spread = [&](const SpreadArgs& a, int nb, float2* scratch) {
     const unsigned ng = cdiv(nb, SP_SB);
-    const size_t seg_lds = sizeof(int) * (2 * (size_t)P_ + 1);
     hipLaunchKernelGGL(spread_kernel, dim3((unsigned)nwork_, ng), dim3(256),
-                       seg_lds, stream, a, work_.as<int4>(),
-                       scratch.as<float2>());
+                       seg_lds, stream, a, work_.as<int4>(), scratch);
     TC_HIP(hipGetLastError());
     if (nfix_) {
       hipLaunchKernelGGL(spread_fixup_kernel, dim3((unsigned)nfix_, ng, 4),
-                         dim3(256), 0, stream, a, fix_.as<int4>(),
-                         scratch.as<float2>());
+                         dim3(256), 0, stream, a, fix_.as<int4>(), scratch);
       TC_HIP(hipGetLastError());
     }
+  };
+  auto scratch_bytes = [&](unsigned ngroups) {
+    return sizeof(float2) * (size_t)std::max(nslots_, 1) * ngroups * SP_SB *
+           SP_T * SP_T;
+  };
+  if (!work_ready_) {
+    count_tiles(stream);
+    const int ntiles = (int)counts_.size();
+    // Points per work item: TOMOCAM_HIP_SPREAD_CHUNK (>= 64), else the
+    // autotuner's choice for this geometry, else (tuning allowed) the
+    // fastest of the rule's choice, 512, 1024 and 2048 on this call's first
+    // batch: best of two timed runs after a warm-up, the rule's choice kept
+    // unless another is >= 2 % faster.  Each work item writes its own tile
+    // or slot, so a timing run only rewrites the fine grid.
+    int chunk = 0;
+    const char* e = std::getenv("TOMOCAM_HIP_SPREAD_CHUNK");
+    if (e && std::atoi(e) >= 64) chunk = std::atoi(e);
+    const auto key = chunk_key(P_, N_, nf_, nm_);
+    if (!chunk) {
+      std::lock_guard<std::mutex> lk(g_chunk_mu);
+      auto it = g_chunk_tuned.find(key);
+      if (it != g_chunk_tuned.end()) chunk = it->second;
+    }
+    if (!chunk && autotune_enabled()) {
+      constexpr float kMargin = 0.02f;
+      const int nb = std::min(B, S);
+      std::vector<int> cands{spread_chunk(ntiles)};
+      for (int k : {512, 1024, 2048})
+        if (k != cands[0]) cands.push_back(k);
+      hipEvent_t e0, e1;
+      TC_HIP(hipEventCreate(&e0));
+      TC_HIP(hipEventCreate(&e1));
+      float best_ms = 0.f;
+      try {
+        for (int k : cands) {
+          make_work(k);
+          DevBuf scr(scratch_bytes(cdiv(nb, SP_SB)));
+          float ms = 0.f;
+          for (int rep = 0; rep < 3; ++rep) {
+            TC_HIP(hipEventRecord(e0, stream));
+            spread(args(0, nb), nb, scr.as<float2>());
+            TC_HIP(hipEventRecord(e1, stream));
+            TC_HIP(hipEventSynchronize(e1));
+            float t = 0.f;
+            TC_HIP(hipEventElapsedTime(&t, e0, e1));
+            if (rep == 1 || (rep > 1 && t < ms)) ms = t;
+          }
+          if (tune_log())
+            std::fprintf(stderr,
+                         "[tomocam] spread tune %d angles x %d, grid %d, %d "
+                         "slices: chunk %d: %.3f ms\n", P_, N_, nf_, nb, k,
+                         (double)ms);
+          if (chunk == 0 || ms < (1.0f - kMargin) * best_ms) {
+            chunk = k;
+            best_ms = ms;
+          }
+        }
+      } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+      }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      std::lock_guard<std::mutex> lk(g_chunk_mu);
+      g_chunk_tuned[key] = chunk;
+    }
+    if (!chunk) chunk = spread_chunk(ntiles);
+    if (chunk != chunk_) make_work(chunk);
+    work_ready_ = true;
+  }
+  DevBuf scratch(scratch_bytes(ngroups_max));
+  for (int s0 = 0; s0 < S; s0 += B) {
+    const int nb = std::min(B, S - s0);
+    const SpreadArgs a = args(s0, nb);
+    spread(a, nb, scratch.as<float2>());
     // only the columns of modes [-nm/2, (nm+1)/2) are read back
     fft2d_pruned(fg, nb, nf_, (nm_ + 1) / 2, nm_ / 2, +1, true, stream);
     const long tot = (long)nb * nm_ * nm_;

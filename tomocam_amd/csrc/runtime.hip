@@ -2,6 +2,7 @@
 // per-kernel-class event timing.
 #include <algorithm>
 #include <atomic>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -97,6 +98,23 @@ ArenaScope::~ArenaScope() {
   }
   a->last = stream_;
   a->pending = true;
+}
+
+static thread_local bool t_autotune = true;
+
+bool autotune_enabled() {
+  const char* e = std::getenv("TOMOCAM_HIP_AUTOTUNE");
+  return t_autotune && !(e && e[0] == '0');
+}
+
+AutotuneScope::AutotuneScope(bool on) : saved_(t_autotune) {
+  t_autotune = t_autotune && on;
+}
+AutotuneScope::~AutotuneScope() { t_autotune = saved_; }
+
+bool tune_log() {
+  const char* e = std::getenv("TOMOCAM_HIP_TUNE_LOG");
+  return e && e[0] == '1';
 }
 
 ArenaPause::ArenaPause() : saved_(t_arena.depth) { t_arena.depth = 0; }

@@ -92,6 +92,24 @@ class ArenaScope {
   bool async_ = false;
   hipStream_t stream_ = nullptr;
 };
+// Autotuning on the calling thread: on unless TOMOCAM_HIP_AUTOTUNE=0 or an
+// AutotuneScope(false) is active (concurrent batch workers, whose timings
+// would include the other workers' kernels).
+bool autotune_enabled();
+class AutotuneScope {
+ public:
+  explicit AutotuneScope(bool on);
+  ~AutotuneScope();
+  AutotuneScope(const AutotuneScope&) = delete;
+  AutotuneScope& operator=(const AutotuneScope&) = delete;
+
+ private:
+  bool saved_;
+};
+// TOMOCAM_HIP_TUNE_LOG=1: the autotuners print each candidate's time to
+// stderr.
+bool tune_log();
+
 // Turns the calling thread's arena off for its lifetime: long-lived objects
 // (FFT twiddle tables, NUFFT and ray-DFT plans) allocate fresh memory,
 // because they fill it with synchronous copies on the null stream, which

@@ -47,13 +47,6 @@ static std::vector<int> toeplitz_ct_candidates(int n) {
   return out;
 }
 
-// TOMOCAM_HIP_TUNE_LOG=1: the autotuners print each candidate's time to
-// stderr
-static bool tune_log() {
-  const char* e = std::getenv("TOMOCAM_HIP_TUNE_LOG");
-  return e && e[0] == '1';
-}
-
 // FFT lengths picked by Psf::tune_length, keyed by (device, n)
 static std::mutex g_len_mu;
 static std::map<std::pair<int, int>, int> g_len_tuned;
