@@ -389,11 +389,22 @@ spread_kernel(SpreadArgs a, const int4* __restrict__ work, float2* scratch) {
           const float4 v23n = pt_val[in][1];
           const int cx = g.x + qb;
           const int cy = g.y + qa;
+          // the point's 8 x weights (lanes 0-7) and 8 y weights (lanes
+          // 8-15), one kernel evaluation per lane, fetched by lane (qa, qb)
+          // from lanes qb and 8 + qa; every lane takes part, so this stays
+          // outside the bounds test
+          const int m = lane & 7;
+          const float dl = __builtin_bit_cast(float, lane < 8 ? g.z : g.w) +
+                           (float)m;
+          const float wl = es_kernel(dl * zscale, a.beta);
+          const float wx = __builtin_bit_cast(
+              float, __builtin_amdgcn_ds_bpermute(
+                         qb << 2, __builtin_bit_cast(int, wl)));
+          const float wy = __builtin_bit_cast(
+              float, __builtin_amdgcn_ds_bpermute(
+                         (8 + qa) << 2, __builtin_bit_cast(int, wl)));
           if (cy >= band0 && cy < band1 && cx >= w.X0 && cx < w.X1) {
-            const float dx = __builtin_bit_cast(float, g.z) + (float)qb;
-            const float dy = __builtin_bit_cast(float, g.w) + (float)qa;
-            const float wgt = es_kernel(dx * zscale, a.beta) *
-                              es_kernel(dy * zscale, a.beta);
+            const float wgt = wx * wy;
             const int cell = (cy - w.Y0) * SP_RS + (cx - w.X0);
             float2 a0 = acc[0][cell], a1 = acc[1][cell];
             float2 a2 = acc[2][cell], a3 = acc[3][cell];
@@ -868,8 +879,11 @@ void PolarNufft::type1_real(const float2* c, float* out, int S, float scale,
     // autotuner's choice for this geometry, else (tuning allowed) the
     // fastest of the rule's choice, 512, 1024 and 2048 on this call's first
     // batch: best of two timed runs after a warm-up, the rule's choice kept
-    // unless another is >= 2 % faster.  Each work item writes its own tile
-    // or slot, so a timing run only rewrites the fine grid.
+    // unless another is >= 5 % faster.  The chunk sets the order in which a
+    // tile's points are summed, so the wide margin keeps near-ties (2048 is
+    // within 0.7-1.3 % of 1024 on large grids) on the rule's choice and the
+    // results the same from process to process.  Each work item writes its
+    // own tile or slot, so a timing run only rewrites the fine grid.
     int chunk = 0;
     const char* e = std::getenv("TOMOCAM_HIP_SPREAD_CHUNK");
     if (e && std::atoi(e) >= 64) chunk = std::atoi(e);
@@ -880,7 +894,7 @@ void PolarNufft::type1_real(const float2* c, float* out, int S, float scale,
       if (it != g_chunk_tuned.end()) chunk = it->second;
     }
     if (!chunk && autotune_enabled()) {
-      constexpr float kMargin = 0.02f;
+      constexpr float kMargin = 0.05f;
       const int nb = std::min(B, S);
       std::vector<int> cands{spread_chunk(ntiles)};
       for (int k : {512, 1024, 2048})
