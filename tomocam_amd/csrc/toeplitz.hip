@@ -156,8 +156,10 @@ int Psf::tune_length(int nslices, hipStream_t stream) {
   }
   // one application of each candidate length on scratch data (one stream,
   // the 144 MB rule's groups), best of two timed runs after a warm-up; the
-  // static rule's choice (first) is replaced only by one >= 2 % faster
-  constexpr float kMargin = 0.02f;
+  // static rule's choice (first) is replaced only by one >= 5 % faster: the
+  // length changes T x in the last bits, so near-ties stay on the rule and
+  // results stay the same from process to process
+  constexpr float kMargin = 0.05f;
   const int S = std::max(1, nslices);
   const long tot = (long)n_ * n_ * S;
   DevBuf x(sizeof(float) * (size_t)tot), out(sizeof(float) * (size_t)tot);
