@@ -421,6 +421,9 @@ static void qggmrf_grid(int S, int R, int C, int* zchunk, unsigned* nblk) {
       best = zc;
     }
   }
+  // TOMOCAM_HIP_STENCIL_ZCHUNK overrides the planes per wave
+  const char* e = std::getenv("TOMOCAM_HIP_STENCIL_ZCHUNK");
+  if (e && std::atoi(e) >= 1) best = std::min(std::atoi(e), S);
   *zchunk = best;
   *nblk = (unsigned)(ntiles * cdiv(S, best));
 }
